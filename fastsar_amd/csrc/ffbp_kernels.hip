@@ -1,0 +1,481 @@
+// Factorized backprojection kernels for gfx950 (CDNA4), float32 arithmetic with float64 phase seeds.
+//
+//   k_prepare       history times the Taylor windows (optional), scan for non-finite samples
+//   k_rot_fir_k     rotate each parent row by every child's phase ramp, decimate along frequency
+//                   (the stage of fastsar/ffbp_cpu.cpp:147-255 and fastsar/ffbp_cuda.py:48-187)
+//   k_fir_p         decimate along pulses (fastsar/ffbp_cpu.cpp:257-293, fastsar/ffbp_cuda.py:190-231)
+//   k_device_phases child rotations of levels >= 2 in float64 (fastsar/ffbp2.py:578-590)
+//   k_final_geom    per tile and final subaperture: unit vector along e1, e2; per tile the aperture mean
+//                   (fastsar/ffbp_cpu.py:187-199, fastsar/ffbp_cuda.py:870-886)
+//   k_final_tiles   the separable per-tile product and the quadratic phase correction, written into the image
+//                   (fastsar/ffbp_cpu.cpp:302-469)
+//
+// Data layout: complex samples interleaved (float2), C order. A level's input is [Np, P, K] (parents, pulses,
+// samples); its output [Np * C, Po, Ko], child-major within each parent. Tiles are numbered in plan order.
+#include "fsh_common.h"
+#include "fsh_kernels.h"
+
+namespace fsh {
+
+// ------------------------------------------------------------------------------------------------ prepare
+
+// x[p, k] = S[p, k] * wp[p] * wk[k] (wp == nullptr: copy); bad: the first pulse holding a NaN or inf
+__global__ __launch_bounds__(256) void k_prepare(const float2* __restrict__ S, int P, int K, const float* __restrict__ wp,
+                                                  const float* __restrict__ wk, float2* __restrict__ x, int* __restrict__ bad)
+{
+    const size_t n = (size_t)P * K;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float2 v = S[i];
+        const int p = (int)(i / K);
+        if (!(isfinite(v.x) && isfinite(v.y))) atomicMin(bad, p);
+        if (wp) {
+            const float g = wp[p] * wk[i - (size_t)p * K];
+            v.x *= g; v.y *= g;
+        }
+        x[i] = v;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ rot_fir_k
+
+// grid (ceil(Ko / JB), ceil(P / PB), Np), block 256. Dynamic LDS: the raw window [PB][W] (float2), the window
+// rotated by the current child's ramp as two planes [PB][WS] (WS = W | 1), the taps [L]; W = D (JB - 1) + L.
+// A raw window is loaded once and serves every child. The ramp of row p, child c: phase c0 + (k - kc) sl cycles,
+// seeded exactly in float64 for each lane (samples l, l + 32, ...) and advanced by a float32 complex recurrence
+// 32 samples apart (at most W / 32 steps).
+__global__ __launch_bounds__(256) void k_rot_fir_k(const float2* __restrict__ x, int P, int K, const float* __restrict__ c0,
+                                                    const float* __restrict__ sl, int Cn, const float* __restrict__ taps, int L,
+                                                    int pl, int D, int Ko, double kc, int JB, int PB, float2* __restrict__ y)
+{
+    extern __shared__ float smem[];
+    const int W = D * (JB - 1) + L;
+    const int WS = W | 1;
+    float2* raw = reinterpret_cast<float2*>(smem);
+    float* zr = smem + 2 * PB * W;
+    float* zi = zr + PB * WS;
+    float* tp = zi + PB * WS;
+    const int tid = threadIdx.x;
+    const int j0 = blockIdx.x * JB, p0 = blockIdx.y * PB, n = blockIdx.z;
+    const int kstart = D * j0 - pl;
+    for (int r = tid; r < L; r += 256) tp[r] = taps[r];
+    const float2* xn = x + (size_t)n * P * K;
+    for (int idx = tid; idx < PB * W; idx += 256) {
+        const int row = idx / W, w = idx - row * W;
+        const int p = p0 + row, k = kstart + w;
+        raw[idx] = (p < P && k >= 0 && k < K) ? xn[(size_t)p * K + k] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const int rl = tid & 31, rrow = tid >> 5;
+    for (int c = 0; c < Cn; ++c) {
+        for (int row = rrow; row < PB; row += 8) {
+            const int p = p0 + row;
+            double a = 0.0, b = 0.0;
+            if (p < P) {
+                a = (double)c0[((size_t)n * Cn + c) * P + p];
+                b = (double)sl[((size_t)n * Cn + c) * P + p];
+            }
+            float wr, wi, sr, si;
+            cis_cyc(fma((double)(kstart + rl) - kc, b, a), wr, wi);
+            cis_cyc(32.0 * b, sr, si);
+            for (int w = rl; w < W; w += 32) {
+                const float2 v = raw[row * W + w];
+                zr[row * WS + w] = fmaf(v.x, wr, -(v.y * wi));
+                zi[row * WS + w] = fmaf(v.x, wi, v.y * wr);
+                const float nr = fmaf(wr, sr, -(wi * si));
+                wi = fmaf(wr, si, wi * sr);
+                wr = nr;
+            }
+        }
+        __syncthreads();
+        for (int o = tid; o < PB * JB; o += 256) {
+            const int pp = o / JB, jj = o - pp * JB;
+            const int p = p0 + pp, j = j0 + jj;
+            if (p < P && j < Ko) {
+                const float* rr = zr + pp * WS + D * jj;
+                const float* ri = zi + pp * WS + D * jj;
+                float ar = 0.f, ai = 0.f;
+                for (int r = 0; r < L; ++r) {
+                    const float t = tp[r];
+                    ar = fmaf(t, rr[r], ar);
+                    ai = fmaf(t, ri[r], ai);
+                }
+                y[(((size_t)n * Cn + c) * P + p) * Ko + j] = make_float2(ar, ai);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ fir_p
+
+// grid (ceil(Ko / 64), ceil(Po / NI), B), block 256 = 64 columns x 4 row groups. Dynamic LDS: the input rows the
+// block's NI outputs read, [rows][64] float2 with rows = D (NI - 1) + L, and the taps.
+__global__ __launch_bounds__(256) void k_fir_p(const float2* __restrict__ y, int P, int Ko, const float* __restrict__ taps,
+                                                int L, int pl, int D, int Po, int NI, float2* __restrict__ z)
+{
+    extern __shared__ float smem[];
+    const int rows = D * (NI - 1) + L;
+    float2* buf = reinterpret_cast<float2*>(smem);
+    float* tp = smem + 2 * rows * 64;
+    const int col = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + col, i0 = blockIdx.y * NI, b = blockIdx.z;
+    const int pstart = D * i0 - pl;
+    const float2* yb = y + (size_t)b * P * Ko;
+    for (int r = q; r < rows; r += 4) {
+        const int p = pstart + r;
+        buf[r * 64 + col] = (p >= 0 && p < P && j < Ko) ? yb[(size_t)p * Ko + j] : make_float2(0.f, 0.f);
+    }
+    for (int r = threadIdx.x; r < L; r += 256) tp[r] = taps[r];
+    __syncthreads();
+    if (j >= Ko) return;
+    for (int io = q; io < NI; io += 4) {
+        const int i = i0 + io;
+        if (i >= Po) break;
+        const float2* src = buf + (io * D) * 64 + col;
+        float ar = 0.f, ai = 0.f;
+        for (int r = 0; r < L; ++r) {
+            const float t = tp[r];
+            const float2 v = src[r * 64];
+            ar = fmaf(t, v.x, ar);
+            ai = fmaf(t, v.y, ai);
+        }
+        z[((size_t)b * Po + i) * Ko + j] = make_float2(ar, ai);
+    }
+}
+
+// the same sums read from global memory, for pulse filters too long for the LDS rows
+__global__ __launch_bounds__(256) void k_fir_p_global(const float2* __restrict__ y, int P, int Ko, const float* __restrict__ taps,
+                                                       int L, int pl, int D, int Po, float2* __restrict__ z)
+{
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63), i = blockIdx.y * 4 + (threadIdx.x >> 6), b = blockIdx.z;
+    if (j >= Ko || i >= Po) return;
+    const float2* yb = y + (size_t)b * P * Ko;
+    const int p0 = D * i - pl;
+    float ar = 0.f, ai = 0.f;
+    for (int r = max(0, -p0); r < L && p0 + r < P; ++r) {
+        const float t = taps[r];
+        const float2 v = yb[(size_t)(p0 + r) * Ko + j];
+        ar = fmaf(t, v.x, ar);
+        ai = fmaf(t, v.y, ai);
+    }
+    z[((size_t)b * Po + i) * Ko + j] = make_float2(ar, ai);
+}
+
+// ------------------------------------------------------------------------------------------------ device phases
+
+// c0, sl [Np, Cn, P]: for parent n at refs[n], child offset d[c], pulse p on the level's track (u, r0):
+// ddr = |ref + d - a| - |ref - a| without cancellation, c0 = wrap(ddr k0c), sl = ddr k1
+__global__ __launch_bounds__(256) void k_device_phases(const double* __restrict__ refs, const double* __restrict__ u,
+                                                        const double* __restrict__ r0, const double* __restrict__ d, int Np,
+                                                        int Cn, int P, double k0c, double k1, float* __restrict__ c0,
+                                                        float* __restrict__ sl)
+{
+    const size_t total = (size_t)Np * Cn * P;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int p = (int)(i % P);
+        const int c = (int)((i / P) % Cn);
+        const int n = (int)(i / ((size_t)P * Cn));
+        const double rx = refs[3 * n], ry = refs[3 * n + 1], rz = refs[3 * n + 2];
+        const double ux = u[3 * p], uy = u[3 * p + 1], uz = u[3 * p + 2];
+        const double dx = d[3 * c], dy = d[3 * c + 1], dz = d[3 * c + 2];
+        const double r = r0[p];
+        const double uc = rx * ux + ry * uy + rz * uz;
+        const double rr = rx * rx + ry * ry + rz * rz;
+        const double wn = r * sqrt(1.0 + (rr - 2.0 * r * uc) / (r * r));
+        const double ud = dx * ux + dy * uy + dz * uz;
+        const double wd = r * ud - (dx * rx + dy * ry + dz * rz);
+        const double num = (dx * dx + dy * dy + dz * dz) - 2.0 * wd;
+        const double ddr = num / (sqrt(wn * wn + num) + wn);
+        c0[i] = (float)wrap_cyc(ddr * k0c);
+        sl[i] = (float)(ddr * k1);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ final geometry
+
+// grid B tiles, block 256: ux, uy [B, Pf] (float64) and per tile (ucx, ucy, rc) [B, 3]
+__global__ __launch_bounds__(256) void k_final_geom(const double* __restrict__ fu, const double* __restrict__ fr0,
+                                                     const double* __restrict__ cen, int Pf, double e1x, double e1y, double e1z,
+                                                     double e2x, double e2y, double e2z, double* __restrict__ ux,
+                                                     double* __restrict__ uy, double* __restrict__ tgeo)
+{
+    __shared__ double red[4][256];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const double enx = e1y * e2z - e1z * e2y, eny = e1z * e2x - e1x * e2z, enz = e1x * e2y - e1y * e2x;
+    const double cx = cen[3 * b], cy = cen[3 * b + 1], cz = cen[3 * b + 2];
+    double sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0;
+    for (int p = tid; p < Pf; p += 256) {
+        const double wx = fr0[p] * fu[3 * p] - cx, wy = fr0[p] * fu[3 * p + 1] - cy, wz = fr0[p] * fu[3 * p + 2] - cz;
+        const double wn = sqrt(wx * wx + wy * wy + wz * wz);
+        const double a = (wx * e1x + wy * e1y + wz * e1z) / wn;
+        const double c = (wx * e2x + wy * e2y + wz * e2z) / wn;
+        const double e = (wx * enx + wy * eny + wz * enz) / wn;
+        ux[(size_t)b * Pf + p] = a;
+        uy[(size_t)b * Pf + p] = c;
+        sx += a; sy += c; sz += e; sw += wn;
+    }
+    red[0][tid] = sx; red[1][tid] = sy; red[2][tid] = sz; red[3][tid] = sw;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s)
+            for (int k = 0; k < 4; ++k) red[k][tid] += red[k][tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double mx = red[0][0] / Pf, my = red[1][0] / Pf, mz = red[2][0] / Pf;
+        const double mn = sqrt(mx * mx + my * my + mz * mz);
+        tgeo[3 * b] = mx / mn;
+        tgeo[3 * b + 1] = my / mn;
+        tgeo[3 * b + 2] = red[3][0] / Pf;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ final tiles
+
+// One workgroup of 256 threads forms NT = 4096 / T^2 tiles (T in 4, 8, 16, 32), each thread a 4 x 4 block of one
+// tile's pixels. The contraction index n = (p, q) over the tile's [Pf, Qf] data runs in chunks of NC = T entries:
+//   A[tile][n][i] = d[p, q] exp(-2 pi i ux_p dlx_i (a0 + a1 q)) (w0 + gx dlx_i)
+//   B[tile][n][j] =         exp(-2 pi i uy_p dly_j (a0 + a1 q)) (1 + gyr dly_j)
+// Each (tile, table, row) is built by one thread along n: seeded in float64 at q = 0 of every subaperture p and
+// advanced by a float32 recurrence over q (exp(-2 pi i u dl a1) per step). The accumulated block is rotated by
+// the per-pixel quadratic phase qc = (dx^2 + dy^2 - (ucx dx + ucy dy)^2) fc2 / (2 rc) and written to the image at
+// the tile's place, cropped to [nx, ny].
+template <int T>
+__global__ __launch_bounds__(256) void k_final_tiles(const float2* __restrict__ data, int B, int Pf, int Qf,
+                                                      const double* __restrict__ ux, const double* __restrict__ uy,
+                                                      const double* __restrict__ tgeo, double spx, double spy, double a0,
+                                                      double a1, double fc2, const float* __restrict__ w0,
+                                                      const float* __restrict__ gx, const float* __restrict__ gyr,
+                                                      const int* __restrict__ tile_xy, int ox, int oy, int nx, int ny,
+                                                      float2* __restrict__ out)
+{
+    extern __shared__ float smem[];
+    constexpr int NT = 4096 / (T * T);
+    constexpr int NC = T;
+    float2* A = reinterpret_cast<float2*>(smem);          // [NT][NC][T]
+    float2* Bt = A + NT * NC * T;                          // [NT][NC][T]
+    const int tid = threadIdx.x;
+    const int bt0 = blockIdx.x * NT;
+    // product role: tile tt, 4 x 4 block (bx, by)
+    const int per_tile = (T * T) / 16;
+    const int tt = tid / per_tile, bi = tid - tt * per_tile;
+    const int bx = bi / (T / 4), by = bi - bx * (T / 4);
+    const bool tok = bt0 + tt < B;
+    // table role: units u = tid + 256 m, u = ((tile * 2 + table) * T + row); 32 / T units per thread
+    constexpr int nunits = 32 / T;
+    float2 w[nunits], st[nunits];       // recurrence state per unit
+    float lw[nunits];
+    const int N = Pf * Qf;
+    float2 acc[4][4];
+    #pragma unroll
+    for (int a = 0; a < 4; ++a)
+        #pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] = make_float2(0.f, 0.f);
+    const double hx = 0.5 * (T - 1);
+    for (int n0 = 0; n0 < N; n0 += NC) {
+        const int nc = min(NC, N - n0);
+        #pragma unroll
+        for (int m = 0; m < nunits; ++m) {
+            const int u = tid + 256 * m;
+            const int row = u % T, tb = (u / T) % 2, tile = u / (2 * T);
+            const int b = bt0 + tile;
+            float2* dst = (tb == 0 ? A : Bt) + (size_t)tile * NC * T + row;
+            const double dl = (row - hx) * (tb == 0 ? spx : spy);
+            for (int s = 0; s < NC; ++s) {
+                const int n = n0 + s;
+                float2 v = make_float2(0.f, 0.f);
+                if (s < nc && b < B) {
+                    const int p = n / Qf, q = n - p * Qf;
+                    if (q == 0 || s == 0) {
+                        // seed at this (p, q): phase -g (a0 + a1 q), step -g a1, g = u_p dl
+                        const size_t bp = (size_t)b * Pf + p;
+                        const double g = (tb == 0 ? ux[bp] : uy[bp]) * dl;
+                        float cs, sn, scs, ssn;
+                        cis_cyc(-(g * fma(a1, (double)q, a0)), cs, sn);
+                        cis_cyc(-(g * a1), scs, ssn);
+                        w[m] = make_float2(cs, sn);
+                        st[m] = make_float2(scs, ssn);
+                        lw[m] = 1.f;
+                        if (w0) lw[m] = tb == 0 ? w0[bp] + gx[bp] * (float)dl : 1.f + gyr[bp] * (float)dl;
+                    }
+                    v = make_float2(w[m].x * lw[m], w[m].y * lw[m]);
+                    if (tb == 0) v = cmul(data[(size_t)b * N + n], v);
+                    w[m] = cmul(w[m], st[m]);
+                }
+                dst[(size_t)s * T] = v;
+            }
+        }
+        __syncthreads();
+        if (tok) {
+            const float2* Ar = A + (size_t)tt * NC * T + 4 * bx;
+            const float2* Br = Bt + (size_t)tt * NC * T + 4 * by;
+            for (int s = 0; s < nc; ++s) {
+                float2 xa[4], yb[4];
+                #pragma unroll
+                for (int a = 0; a < 4; ++a) { xa[a] = Ar[(size_t)s * T + a]; yb[a] = Br[(size_t)s * T + a]; }
+                #pragma unroll
+                for (int a = 0; a < 4; ++a)
+                    #pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        acc[a][c].x = fmaf(xa[a].x, yb[c].x, acc[a][c].x);
+                        acc[a][c].x = fmaf(-xa[a].y, yb[c].y, acc[a][c].x);
+                        acc[a][c].y = fmaf(xa[a].x, yb[c].y, acc[a][c].y);
+                        acc[a][c].y = fmaf(xa[a].y, yb[c].x, acc[a][c].y);
+                    }
+            }
+        }
+        __syncthreads();
+    }
+    if (!tok) return;
+    const int b = bt0 + tt;
+    const double cx = tgeo[3 * b], cy = tgeo[3 * b + 1], f = fc2 / (2.0 * tgeo[3 * b + 2]);
+    const int X0 = tile_xy[2 * b] - ox, Y0 = tile_xy[2 * b + 1] - oy;
+    #pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int i = 4 * bx + a;
+        const int X = X0 + i;
+        const double dx = (i - hx) * spx;
+        #pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int j = 4 * by + c;
+            const int Y = Y0 + j;
+            if (X < 0 || X >= nx || Y < 0 || Y >= ny) continue;
+            const double dy = (j - hx) * spy;
+            const double los = cx * dx + cy * dy;
+            float cs, sn;
+            cis_cyc((dx * dx + dy * dy - los * los) * f, cs, sn);
+            out[(size_t)X * ny + Y] = cmul(acc[a][c], make_float2(cs, sn));
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ launchers
+
+static int lds_limit()
+{
+    static int lim = -1;
+    if (lim < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            v = 65536;
+        lim = v;
+    }
+    return lim;
+}
+
+int launch_prepare(const float2* S, int P, int K, const float* wp, const float* wk, float2* x, int* bad, hipStream_t s)
+{
+    const size_t n = (size_t)P * K;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(k_prepare, dim3(blocks), dim3(256), 0, s, S, P, K, wp, wk, x, bad);
+    FSH_TRY(hipGetLastError());
+    return OK;
+}
+
+int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
+                     int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s)
+{
+    if (Np < 1 || P < 1 || K < 1 || Cn < 1 || L < 1 || D < 1 || Ko < 1 || pl < 0) {
+        set_error("rot_fir_k: invalid shape");
+        return E_INVALID;
+    }
+    const int PB = 8;
+    const int budget = std::min(lds_limit(), 65536);
+    int JB = 64;
+    auto need = [&](int jb) { const int W = D * (jb - 1) + L; return 2 * PB * W * 4 + 2 * PB * (W | 1) * 4 + L * 4; };
+    while (JB > 4 && need(JB) > budget) JB /= 2;
+    if (need(JB) > budget) {
+        set_error("rot_fir_k: the filter window does not fit in LDS");
+        return E_INVALID;
+    }
+    const long gy = (P + PB - 1) / PB;
+    if (gy > 2147483647L) { set_error("rot_fir_k: too many pulses"); return E_INVALID; }
+    for (int n0 = 0; n0 < Np; n0 += 65535) {
+        const int nn = std::min(65535, Np - n0);
+        dim3 grid((Ko + JB - 1) / JB, (unsigned)gy, nn);
+        hipLaunchKernelGGL(k_rot_fir_k, grid, dim3(256), need(JB), s, x + (size_t)n0 * P * K, P, K, c0 + (size_t)n0 * Cn * P,
+                           sl + (size_t)n0 * Cn * P, Cn, taps, L, pl, D, Ko, kc, JB, PB, y + (size_t)n0 * Cn * P * Ko);
+        FSH_TRY(hipGetLastError());
+    }
+    return OK;
+}
+
+int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float2* z, hipStream_t s)
+{
+    if (B < 1 || P < 1 || Ko < 1 || L < 1 || D < 1 || Po < 1 || pl < 0) {
+        set_error("fir_p: invalid shape");
+        return E_INVALID;
+    }
+    const int budget = std::min(lds_limit(), 65536);
+    const int max_rows = (budget - 4 * L) / 512;
+    if (L <= max_rows) {
+        int NI = std::max(1, (max_rows - L) / D + 1);
+        NI = std::min(NI, Po);
+        NI = std::min(NI, 64);
+        const int rows = D * (NI - 1) + L;
+        for (int b0 = 0; b0 < B; b0 += 65535) {
+            const int nb = std::min(65535, B - b0);
+            dim3 grid((Ko + 63) / 64, (Po + NI - 1) / NI, nb);
+            hipLaunchKernelGGL(k_fir_p, grid, dim3(256), rows * 512 + 4 * L, s, y + (size_t)b0 * P * Ko, P, Ko, taps, L, pl, D,
+                               Po, NI, z + (size_t)b0 * Po * Ko);
+            FSH_TRY(hipGetLastError());
+        }
+    } else {
+        for (int b0 = 0; b0 < B; b0 += 65535) {
+            const int nb = std::min(65535, B - b0);
+            dim3 grid((Ko + 63) / 64, (Po + 3) / 4, nb);
+            hipLaunchKernelGGL(k_fir_p_global, grid, dim3(256), 0, s, y + (size_t)b0 * P * Ko, P, Ko, taps, L, pl, D, Po,
+                               z + (size_t)b0 * Po * Ko);
+            FSH_TRY(hipGetLastError());
+        }
+    }
+    return OK;
+}
+
+int launch_device_phases(const double* refs, const double* u, const double* r0, const double* d, int Np, int Cn, int P,
+                         double k0c, double k1, float* c0, float* sl, hipStream_t s)
+{
+    const size_t n = (size_t)Np * Cn * P;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(k_device_phases, dim3(blocks), dim3(256), 0, s, refs, u, r0, d, Np, Cn, P, k0c, k1, c0, sl);
+    FSH_TRY(hipGetLastError());
+    return OK;
+}
+
+int launch_final_geom(const double* fu, const double* fr0, const double* cen, int B, int Pf, const double* e1, const double* e2,
+                      double* ux, double* uy, double* tgeo, hipStream_t s)
+{
+    if (B < 1 || Pf < 1) { set_error("final_geom: invalid shape"); return E_INVALID; }
+    hipLaunchKernelGGL(k_final_geom, dim3(B), dim3(256), 0, s, fu, fr0, cen, Pf, e1[0], e1[1], e1[2], e2[0], e2[1], e2[2], ux,
+                       uy, tgeo);
+    FSH_TRY(hipGetLastError());
+    return OK;
+}
+
+int launch_final_tiles(const float2* data, int B, int Pf, int Qf, int T, const double* ux, const double* uy,
+                       const double* tgeo, double spx, double spy, double a0, double a1, double fc2, const float* w0,
+                       const float* gx, const float* gyr, const int* tile_xy, int ox, int oy, int nx, int ny, float2* out,
+                       hipStream_t s)
+{
+    if (!(T == 4 || T == 8 || T == 16 || T == 32)) {
+        set_error("final_tiles: T must be 4, 8, 16 or 32");
+        return E_INVALID;
+    }
+    if (B < 1 || Pf < 1 || Qf < 1) { set_error("final_tiles: invalid shape"); return E_INVALID; }
+    const int NT = 4096 / (T * T);
+    const size_t lds = (size_t)2 * NT * T * T * 8;
+    const long blocks = (B + NT - 1) / NT;
+#define FSH_FINAL(TT) hipLaunchKernelGGL(k_final_tiles<TT>, dim3((unsigned)blocks), dim3(256), lds, s, data, B, Pf, Qf, ux, uy, \
+                                         tgeo, spx, spy, a0, a1, fc2, w0, gx, gyr, tile_xy, ox, oy, nx, ny, out)
+    switch (T) {
+        case 4: FSH_FINAL(4); break;
+        case 8: FSH_FINAL(8); break;
+        case 16: FSH_FINAL(16); break;
+        default: FSH_FINAL(32); break;
+    }
+#undef FSH_FINAL
+    FSH_TRY(hipGetLastError());
+    return OK;
+}
+
+}  // namespace fsh

@@ -1,0 +1,511 @@
+// The C ABI (include/fsh.h): errors, device memory, stage entry points and the native FFBP former.
+//
+// The former holds a plan's constants on the device and runs the whole formation on its own stream:
+// window and finite-sample scan, then the first-level children in groups (fastsar/ffbp_cpu.py:238-291,
+// fastsar/ffbp_cuda.py:1139-1198), each group carried through the remaining levels and the final stage, which
+// writes its tiles straight into the image. Intermediates live in two ping-pong buffers sized for the group.
+#include "fsh_common.h"
+#include "fsh_kernels.h"
+#include "../../include/fsh.h"
+
+#include <algorithm>
+#include <atomic>
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace fsh {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const std::string& last_error() { return g_err; }
+
+int hip_status(hipError_t e, const char* what)
+{
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? E_NOMEM : E_RUNTIME;
+}
+
+static std::atomic<long> g_alloc_count{0};
+
+int dev_alloc(void** p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    const char* fail = getenv("FSH_FAIL_ALLOC");
+    const long n = ++g_alloc_count;
+    if (fail && atol(fail) > 0 && n == atol(fail)) {
+        set_error("device allocation failed (FSH_FAIL_ALLOC)");
+        return E_NOMEM;
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        char buf[160];
+        snprintf(buf, sizeof buf, "out of device memory allocating %.3f GB", bytes / 1e9);
+        set_error(buf);
+        return E_NOMEM;
+    }
+    return OK;
+}
+
+void dev_free(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+__global__ void k_fill_int(int* p, int n, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int launch_fill_int(int* p, int n, int value, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, p, n, value);
+    FSH_TRY(hipGetLastError());
+    return OK;
+}
+
+// ------------------------------------------------------------------------------------------------ former
+
+template <class T>
+static int upload(T** dst, const T* src, size_t count)
+{
+    *dst = nullptr;
+    if (!src || count == 0) return OK;
+    int st = dev_alloc((void**)dst, count * sizeof(T));
+    if (st) return st;
+    FSH_TRY(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return OK;
+}
+
+struct Level {
+    fsh_level_desc h;              // shapes (host pointers not kept)
+    float* taps_k = nullptr;
+    float* taps_p = nullptr;
+    float* c0 = nullptr;
+    float* sl = nullptr;
+    double* u = nullptr;
+    double* r0 = nullptr;
+    double* d = nullptr;
+    double* ref = nullptr;
+    long long below = 1;           // tiles per parent... children per first-level child at this level's input
+};
+
+enum { ST_PREP, ST_ROT, ST_FIRP, ST_PHASE, ST_GEOM, ST_FINAL, ST_N };
+
+struct Former {
+    fsh_plan_desc p;
+    std::vector<Level> lv;
+    hipStream_t stream = nullptr;
+    float* one = nullptr;          // the single unit tap of an undecimated axis
+    float* wp = nullptr;
+    float* wk = nullptr;
+    double* fu = nullptr;
+    double* fr0 = nullptr;
+    double* fcen = nullptr;
+    int* tile_xy = nullptr;
+    float* w0 = nullptr;
+    float* gx = nullptr;
+    float* gyr = nullptr;
+    double e1[3], e2[3];
+    // per call
+    float2* hist = nullptr;        // the history as uploaded (host input)
+    float2* x = nullptr;           // windowed history
+    int* bad = nullptr;
+    float2* image = nullptr;
+    // per group
+    int ng_req = 0, ng = 0;
+    float2* buf[2] = {nullptr, nullptr};
+    size_t bufsz = 0;
+    float* c0b = nullptr;
+    float* slb = nullptr;
+    size_t phsz = 0;
+    double* ux = nullptr;
+    double* uy = nullptr;
+    double* tgeo = nullptr;
+    size_t geosz = 0;
+    size_t bytes = 0;
+    bool profile = false;
+    float stage_ms[ST_N] = {0};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+
+    ~Former() { release_group(); release_all(); }
+
+    void release_group()
+    {
+        dev_free(buf[0]); dev_free(buf[1]); dev_free(c0b); dev_free(slb); dev_free(ux); dev_free(uy); dev_free(tgeo);
+        buf[0] = buf[1] = nullptr; c0b = slb = nullptr; ux = uy = tgeo = nullptr;
+        bufsz = phsz = geosz = 0;
+        ng = 0;
+    }
+
+    void release_all()
+    {
+        for (auto& l : lv) {
+            dev_free(l.taps_k); dev_free(l.taps_p); dev_free(l.c0); dev_free(l.sl); dev_free(l.u); dev_free(l.r0);
+            dev_free(l.d); dev_free(l.ref);
+        }
+        lv.clear();
+        dev_free(one); dev_free(wp); dev_free(wk); dev_free(fu); dev_free(fr0); dev_free(fcen); dev_free(tile_xy);
+        dev_free(w0); dev_free(gx); dev_free(gyr); dev_free(hist); dev_free(x); dev_free(bad); dev_free(image);
+        if (ev[0]) (void)hipEventDestroy(ev[0]);
+        if (ev[1]) (void)hipEventDestroy(ev[1]);
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+
+    // sizes of a group of g first-level children
+    void group_sizes(int g, size_t& maxbuf, size_t& maxph, size_t& tiles) const
+    {
+        maxbuf = 0; maxph = 0;
+        long long parents = 1;     // parents at the current level within the group (level 0: the history)
+        for (size_t l = 0; l < lv.size(); ++l) {
+            const fsh_level_desc& h = lv[l].h;
+            const long long ch = (l == 0) ? g : (long long)h.C;
+            const long long outs = parents * ch;
+            maxbuf = std::max(maxbuf, (size_t)(outs * h.P * h.Ko));
+            if (h.Dp > 1) maxbuf = std::max(maxbuf, (size_t)(outs * h.Po * h.Ko));
+            if (l >= 2) maxph = std::max(maxph, (size_t)(parents * h.C * h.P));
+            parents = outs;
+        }
+        tiles = (size_t)parents;
+    }
+
+    int alloc_group(int g)
+    {
+        release_group();
+        size_t mb, mp, tiles;
+        group_sizes(g, mb, mp, tiles);
+        int st;
+        if ((st = dev_alloc((void**)&buf[0], mb * sizeof(float2)))) return st;
+        if ((st = dev_alloc((void**)&buf[1], mb * sizeof(float2)))) return st;
+        if (mp) {
+            if ((st = dev_alloc((void**)&c0b, mp * sizeof(float)))) return st;
+            if ((st = dev_alloc((void**)&slb, mp * sizeof(float)))) return st;
+        }
+        if ((st = dev_alloc((void**)&ux, tiles * p.Pf * sizeof(double)))) return st;
+        if ((st = dev_alloc((void**)&uy, tiles * p.Pf * sizeof(double)))) return st;
+        if ((st = dev_alloc((void**)&tgeo, tiles * 3 * sizeof(double)))) return st;
+        bufsz = mb; phsz = mp; geosz = tiles;
+        ng = g;
+        return OK;
+    }
+
+    void mark_begin()
+    {
+        if (profile) (void)hipEventRecord(ev[0], stream);
+    }
+
+    void mark_end(int stage)
+    {
+        if (!profile) return;
+        (void)hipEventRecord(ev[1], stream);
+        (void)hipEventSynchronize(ev[1]);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        stage_ms[stage] += ms;
+    }
+
+    int run_group(int g0, int gn)
+    {
+        int st;
+        const float2* cur = x;
+        int curbuf = -1;
+        long long parents = 1;
+        long long first_parent = 0;    // index of the group's first parent at the current level, plan order
+        for (size_t l = 0; l < lv.size(); ++l) {
+            const Level& L = lv[l];
+            const fsh_level_desc& h = L.h;
+            const int ch = (l == 0) ? gn : h.C;
+            const int yb = (curbuf == 0) ? 1 : 0;
+            const float* c0;
+            const float* sl;
+            if (l == 0) {
+                c0 = L.c0 + (size_t)g0 * h.P;
+                sl = L.sl + (size_t)g0 * h.P;
+            } else if (L.c0) {
+                c0 = L.c0 + (size_t)first_parent * h.C * h.P;
+                sl = L.sl + (size_t)first_parent * h.C * h.P;
+            } else {
+                mark_begin();
+                if ((st = launch_device_phases(L.ref + 3 * first_parent, L.u, L.r0, L.d, (int)parents, h.C, h.P, h.k0c, h.k1,
+                                               c0b, slb, stream)))
+                    return st;
+                mark_end(ST_PHASE);
+                c0 = c0b;
+                sl = slb;
+            }
+            mark_begin();
+            if ((st = launch_rot_fir_k(cur, (int)parents, h.P, h.K, c0, sl, ch, h.Dk > 1 ? L.taps_k : one, h.Dk > 1 ? h.Lk : 1,
+                                       h.Dk > 1 ? h.plk : 0, h.Dk, h.Ko, h.kc, buf[yb], stream)))
+                return st;
+            mark_end(ST_ROT);
+            cur = buf[yb];
+            curbuf = yb;
+            if (h.Dp > 1) {
+                mark_begin();
+                if ((st = launch_fir_p(buf[yb], (int)(parents * ch), h.P, h.Ko, L.taps_p, h.Lp, h.plp, h.Dp, h.Po, buf[1 - yb],
+                                       stream)))
+                    return st;
+                mark_end(ST_FIRP);
+                cur = buf[1 - yb];
+                curbuf = 1 - yb;
+            }
+            first_parent = (l == 0) ? g0 : first_parent * h.C;
+            parents *= ch;
+        }
+        const long long tiles = parents, t0 = first_parent;
+        mark_begin();
+        if ((st = launch_final_geom(fu, fr0, fcen + 3 * t0, (int)tiles, p.Pf, e1, e2, ux, uy, tgeo, stream))) return st;
+        mark_end(ST_GEOM);
+        mark_begin();
+        if ((st = launch_final_tiles(cur, (int)tiles, p.Pf, p.Qf, p.T, ux, uy, tgeo, p.spx, p.spy, p.a0, p.a1, p.fc2,
+                                     w0 ? w0 + t0 * p.Pf : nullptr, gx ? gx + t0 * p.Pf : nullptr,
+                                     gyr ? gyr + t0 * p.Pf : nullptr, tile_xy + 2 * t0, p.ox, p.oy, p.nx, p.ny, image, stream)))
+            return st;
+        mark_end(ST_FINAL);
+        return OK;
+    }
+};
+
+}  // namespace fsh
+
+using namespace fsh;
+
+extern "C" {
+
+int fsh_version(void) { return 100; }
+
+int fsh_last_error(char* buf, int n)
+{
+    if (!buf || n <= 0) return E_INVALID;
+    snprintf(buf, (size_t)n, "%s", last_error().c_str());
+    return OK;
+}
+
+int fsh_device_info(char* buf, int n)
+{
+    int dev = 0;
+    FSH_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t pr;
+    FSH_TRY(hipGetDeviceProperties(&pr, dev));
+    if (buf && n > 0)
+        snprintf(buf, (size_t)n, "%s|%s|cu=%d|lds=%zu|mem=%zu", pr.name, pr.gcnArchName, pr.multiProcessorCount,
+                 (size_t)pr.sharedMemPerBlock, (size_t)pr.totalGlobalMem);
+    return OK;
+}
+
+int fsh_malloc(void** ptr, size_t bytes) { return dev_alloc(ptr, bytes); }
+int fsh_free(void* ptr) { dev_free(ptr); return OK; }
+int fsh_memcpy_h2d(void* dst, const void* src, size_t bytes) { FSH_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); return OK; }
+int fsh_memcpy_d2h(void* dst, const void* src, size_t bytes) { FSH_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); return OK; }
+int fsh_memcpy_d2d(void* dst, const void* src, size_t bytes) { FSH_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice)); return OK; }
+int fsh_memset(void* dst, int value, size_t bytes) { FSH_TRY(hipMemset(dst, value, bytes)); return OK; }
+int fsh_synchronize(void) { FSH_TRY(hipDeviceSynchronize()); return OK; }
+
+int fsh_rot_fir_k(const float* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps, int L,
+                  int pl, int D, int Ko, double kc, float* y, void* stream)
+{
+    return launch_rot_fir_k((const float2*)x, Np, P, K, c0, sl, Cn, taps, L, pl, D, Ko, kc, (float2*)y, (hipStream_t)stream);
+}
+
+int fsh_fir_p(const float* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float* z, void* stream)
+{
+    return launch_fir_p((const float2*)y, B, P, Ko, taps, L, pl, D, Po, (float2*)z, (hipStream_t)stream);
+}
+
+int fsh_device_phases(const double* refs, const double* u, const double* r0, const double* d, int Np, int Cn, int P,
+                      double k0c, double k1, float* c0, float* sl, void* stream)
+{
+    if (Np < 1 || Cn < 1 || P < 1) { set_error("device_phases: invalid shape"); return E_INVALID; }
+    return launch_device_phases(refs, u, r0, d, Np, Cn, P, k0c, k1, c0, sl, (hipStream_t)stream);
+}
+
+int fsh_final_geom(const double* fu, const double* fr0, const double* cen, int B, int Pf, const double* e1, const double* e2,
+                   double* ux, double* uy, double* tgeo, void* stream)
+{
+    return launch_final_geom(fu, fr0, cen, B, Pf, e1, e2, ux, uy, tgeo, (hipStream_t)stream);
+}
+
+int fsh_final_tiles(const float* d, int B, int Pf, int Qf, int T, const double* ux, const double* uy, const double* tgeo,
+                    double spx, double spy, double a0, double a1, double fc2, const float* w0, const float* gx,
+                    const float* gyr, const int* tile_xy, int ox, int oy, int nx, int ny, float* out, void* stream)
+{
+    return launch_final_tiles((const float2*)d, B, Pf, Qf, T, ux, uy, tgeo, spx, spy, a0, a1, fc2, w0, gx, gyr, tile_xy, ox,
+                              oy, nx, ny, (float2*)out, (hipStream_t)stream);
+}
+
+int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, void** handle)
+{
+    if (!plan || !levels || !handle) { set_error("ffbp_create: null argument"); return E_INVALID; }
+    *handle = nullptr;
+    const fsh_plan_desc& p = *plan;
+    if (p.nlev < 1 || p.nlev > 12 || p.Pf < 1 || p.Qf < 1 || p.ntiles < 1 || p.nx < 1 || p.ny < 1) {
+        set_error("ffbp_create: invalid plan");
+        return E_INVALID;
+    }
+    if (!(p.T == 4 || p.T == 8 || p.T == 16 || p.T == 32)) {
+        set_error("ffbp_create: the hip final stage takes T = 4, 8, 16 or 32");
+        return E_INVALID;
+    }
+    Former* f = new Former();
+    f->p = p;
+    memcpy(f->e1, p.e1, sizeof f->e1);
+    memcpy(f->e2, p.e2, sizeof f->e2);
+    f->profile = getenv("FSH_PROFILE") && atoi(getenv("FSH_PROFILE")) > 0;
+    int st = OK;
+    auto fail = [&](int s) { delete f; return s; };
+    if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("ffbp_create: stream creation failed");
+        return fail(E_RUNTIME);
+    }
+    if (hipEventCreate(&f->ev[0]) != hipSuccess || hipEventCreate(&f->ev[1]) != hipSuccess) return fail(E_RUNTIME);
+    const float unit = 1.0f;
+    if ((st = upload(&f->one, &unit, 1))) return fail(st);
+    long long children = 1;
+    for (int l = 0; l < p.nlev; ++l) {
+        const fsh_level_desc& h = levels[l];
+        if (h.C < 1 || h.Dk < 1 || h.Dp < 1 || h.K < 1 || h.P < 1 || h.Ko < 1 || h.Po < 1 || h.nparents < 1 ||
+            (h.Dk > 1 && (!h.taps_k || h.Lk < 1)) || (h.Dp > 1 && (!h.taps_p || h.Lp < 1)) ||
+            (!h.c0 && (!h.u || !h.r0 || !h.d || !h.ref)) || (l == 0 && !h.c0)) {
+            set_error("ffbp_create: invalid level " + std::to_string(l));
+            return fail(E_INVALID);
+        }
+        Level L;
+        L.h = h;
+        if ((st = upload(&L.taps_k, h.taps_k, h.Dk > 1 ? (size_t)h.Lk : 0))) return fail(st);
+        if ((st = upload(&L.taps_p, h.taps_p, h.Dp > 1 ? (size_t)h.Lp : 0))) return fail(st);
+        if (h.c0) {
+            const size_t n = (size_t)h.nparents * h.C * h.P;
+            if ((st = upload(&L.c0, h.c0, n)) || (st = upload(&L.sl, h.sl, n))) return fail(st);
+        } else {
+            if ((st = upload(&L.u, h.u, (size_t)h.P * 3)) || (st = upload(&L.r0, h.r0, (size_t)h.P)) ||
+                (st = upload(&L.d, h.d, (size_t)h.C * 3)) || (st = upload(&L.ref, h.ref, (size_t)h.nparents * 3)))
+                return fail(st);
+        }
+        L.below = children;
+        children *= h.C;
+        L.h.taps_k = L.h.taps_p = L.h.c0 = L.h.sl = nullptr;
+        L.h.u = L.h.r0 = L.h.d = L.h.ref = nullptr;
+        f->lv.push_back(L);
+    }
+    if (children != p.ntiles) {
+        set_error("ffbp_create: the levels' children do not multiply to the tile count");
+        return fail(E_INVALID);
+    }
+    if ((st = upload(&f->wp, p.wp, p.wp ? (size_t)p.P : 0)) || (st = upload(&f->wk, p.wk, p.wk ? (size_t)p.K : 0)) ||
+        (st = upload(&f->fu, p.fu, (size_t)p.Pf * 3)) || (st = upload(&f->fr0, p.fr0, (size_t)p.Pf)) ||
+        (st = upload(&f->fcen, p.fcen, (size_t)p.ntiles * 3)) || (st = upload(&f->tile_xy, p.tile_xy, (size_t)p.ntiles * 2)))
+        return fail(st);
+    if (p.w0) {
+        const size_t n = (size_t)p.ntiles * p.Pf;
+        if ((st = upload(&f->w0, p.w0, n)) || (st = upload(&f->gx, p.gx, n)) || (st = upload(&f->gyr, p.gyr, n))) return fail(st);
+    }
+    f->p.wp = f->p.wk = nullptr;
+    f->p.fu = f->p.fr0 = f->p.fcen = nullptr;
+    f->p.tile_xy = nullptr;
+    f->p.w0 = f->p.gx = f->p.gyr = nullptr;
+    if ((st = dev_alloc((void**)&f->x, (size_t)p.P * p.K * sizeof(float2)))) return fail(st);
+    if ((st = dev_alloc((void**)&f->bad, sizeof(int)))) return fail(st);
+    if ((st = dev_alloc((void**)&f->image, (size_t)p.nx * p.ny * sizeof(float2)))) return fail(st);
+    *handle = f;
+    return OK;
+}
+
+int fsh_ffbp_set_group(void* handle, int ng)
+{
+    if (!handle || ng < 0) { set_error("ffbp_set_group: invalid argument"); return E_INVALID; }
+    static_cast<Former*>(handle)->ng_req = ng;
+    return OK;
+}
+
+int fsh_ffbp_info(void* handle, int* ng_used, size_t* device_bytes)
+{
+    if (!handle) { set_error("ffbp_info: null handle"); return E_INVALID; }
+    Former* f = static_cast<Former*>(handle);
+    if (ng_used) *ng_used = f->ng;
+    if (device_bytes) {
+        size_t mb, mp, tiles;
+        f->group_sizes(std::max(f->ng, 1), mb, mp, tiles);
+        *device_bytes = (size_t)f->p.P * f->p.K * 8 * (f->hist ? 2 : 1) + (size_t)f->p.nx * f->p.ny * 8 + 2 * mb * 8 + 2 * mp * 4 +
+                        tiles * f->p.Pf * 16;
+    }
+    return OK;
+}
+
+int fsh_ffbp_stage_ms(void* handle, float* ms, int n)
+{
+    if (!handle || !ms) { set_error("ffbp_stage_ms: null argument"); return E_INVALID; }
+    Former* f = static_cast<Former*>(handle);
+    for (int i = 0; i < n && i < ST_N; ++i) ms[i] = f->stage_ms[i];
+    return OK;
+}
+
+int fsh_ffbp_form(void* handle, const void* S, int S_on_device, void* image, int image_on_device, int* bad_pulse)
+{
+    if (!handle || !S || !image) { set_error("ffbp_form: null argument"); return E_INVALID; }
+    Former* f = static_cast<Former*>(handle);
+    const fsh_plan_desc& p = f->p;
+    int st;
+    for (int i = 0; i < ST_N; ++i) f->stage_ms[i] = 0.f;
+    const float2* src = (const float2*)S;
+    if (!S_on_device) {
+        if (!f->hist && (st = dev_alloc((void**)&f->hist, (size_t)p.P * p.K * sizeof(float2)))) return st;
+        FSH_TRY(hipMemcpyAsync(f->hist, S, (size_t)p.P * p.K * sizeof(float2), hipMemcpyHostToDevice, f->stream));
+        src = f->hist;
+    }
+    f->mark_begin();
+    if ((st = launch_fill_int(f->bad, 1, INT_MAX, f->stream))) return st;
+    if ((st = launch_prepare(src, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream))) return st;
+    f->mark_end(ST_PREP);
+    int badh = INT_MAX;
+    FSH_TRY(hipMemcpyAsync(&badh, f->bad, sizeof(int), hipMemcpyDeviceToHost, f->stream));
+    FSH_TRY(hipStreamSynchronize(f->stream));
+    if (badh != INT_MAX) {
+        if (bad_pulse) *bad_pulse = badh;
+        char buf[200];
+        snprintf(buf, sizeof buf, "phase history has non-finite samples (NaN or inf), the first in pulse %d", badh);
+        set_error(buf);
+        return E_NONFINITE;
+    }
+    // the group: requested, else up to 8, halved while the buffers do not fit
+    const int G = f->lv[0].h.C;
+    int want = f->ng_req > 0 ? std::min(f->ng_req, G) : std::min(8, G);
+    if (f->ng != want) {
+        while (true) {
+            st = f->alloc_group(want);
+            if (st == OK) break;
+            f->release_group();
+            if (st != E_NOMEM || want == 1 || f->ng_req > 0) return st;
+            want = std::max(1, want / 2);
+        }
+    }
+    for (int g0 = 0; g0 < G; g0 += f->ng) {
+        const int gn = std::min(f->ng, G - g0);
+        if ((st = f->run_group(g0, gn))) return st;
+    }
+    if (image_on_device) {
+        FSH_TRY(hipMemcpyAsync(image, f->image, (size_t)p.nx * p.ny * sizeof(float2), hipMemcpyDeviceToDevice, f->stream));
+    } else {
+        FSH_TRY(hipMemcpyAsync(image, f->image, (size_t)p.nx * p.ny * sizeof(float2), hipMemcpyDeviceToHost, f->stream));
+    }
+    FSH_TRY(hipStreamSynchronize(f->stream));
+    return OK;
+}
+
+int fsh_ffbp_destroy(void* handle)
+{
+    if (handle) {
+        Former* f = static_cast<Former*>(handle);
+        if (f->stream) (void)hipStreamSynchronize(f->stream);
+        delete f;
+    }
+    return OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,42 @@
+// Shared definitions of the HIP image-formation library (gfx950 / CDNA4).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+namespace fsh {
+
+constexpr double TWO_PI = 6.283185307179586476925286766559;
+
+enum Status : int { OK = 0, E_NOMEM = 1, E_INVALID = 2, E_NONFINITE = 3, E_RUNTIME = 4 };
+
+// last error message of the calling thread (fsh_last_error)
+void set_error(const std::string& msg);
+const std::string& last_error();
+
+// checks a HIP call; on failure records the message and returns E_RUNTIME (E_NOMEM for allocation failures)
+int hip_status(hipError_t e, const char* what);
+
+#define FSH_TRY(call)                                                   \
+    do {                                                                \
+        hipError_t e_ = (call);                                         \
+        if (e_ != hipSuccess) return ::fsh::hip_status(e_, #call);      \
+    } while (0)
+
+// wrap a phase given in cycles to [-1/2, 1/2]
+__device__ __forceinline__ double wrap_cyc(double c) { return c - rint(c); }
+
+// cos and sin of 2 pi c for c in cycles (wrapped in float64 first, evaluated in float32 by ROCm's device math)
+__device__ __forceinline__ void cis_cyc(double c, float& cs, float& sn)
+{
+    const float w = (float)wrap_cyc(c);
+    sincospif(2.0f * w, &sn, &cs);
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
+
+}  // namespace fsh

@@ -1,0 +1,27 @@
+// Launchers of the factorized-backprojection kernels (device pointers, one stream). Each validates its shapes,
+// chooses the launch geometry and returns a status (fsh_common.h).
+#pragma once
+#include "fsh_common.h"
+
+namespace fsh {
+
+int launch_prepare(const float2* S, int P, int K, const float* wp, const float* wk, float2* x, int* bad, hipStream_t s);
+int launch_fill_int(int* p, int n, int value, hipStream_t s);
+int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
+                     int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s);
+int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float2* z,
+                 hipStream_t s);
+int launch_device_phases(const double* refs, const double* u, const double* r0, const double* d, int Np, int Cn, int P,
+                         double k0c, double k1, float* c0, float* sl, hipStream_t s);
+int launch_final_geom(const double* fu, const double* fr0, const double* cen, int B, int Pf, const double* e1,
+                      const double* e2, double* ux, double* uy, double* tgeo, hipStream_t s);
+int launch_final_tiles(const float2* data, int B, int Pf, int Qf, int T, const double* ux, const double* uy,
+                       const double* tgeo, double spx, double spy, double a0, double a1, double fc2, const float* w0,
+                       const float* gx, const float* gyr, const int* tile_xy, int ox, int oy, int nx, int ny, float2* out,
+                       hipStream_t s);
+
+// device allocation through one helper, so that a test can make the n-th allocation fail (FSH_FAIL_ALLOC=n)
+int dev_alloc(void** p, size_t bytes);
+void dev_free(void* p);
+
+}  // namespace fsh

@@ -1,0 +1,134 @@
+/* fsh: MI355X (gfx950) image formation for FastSAR's former API - the C ABI.
+ *
+ * A drop-in GPU backend for the reference's image-formation path: factorized backprojection (ImageFormer,
+ * form_image) and exact backprojection (ExactFormer). Plain C types only; complex samples are interleaved float
+ * pairs (numpy complex64 memory). Every function returns a status:
+ *   0 FSH_OK, 1 FSH_ENOMEM (out of device memory: MemoryError), 2 FSH_EINVAL (ValueError),
+ *   3 FSH_ENONFINITE (NaN or inf in the phase history: ValueError naming the pulse), 4 FSH_ERUNTIME (HIP error);
+ * fsh_last_error() gives the calling thread's message. The caller owns host buffers; the library owns the device
+ * buffers of a former. A former handle is not re-entrant: one call at a time per handle.
+ *
+ * The reference binds its CPU kernels with ctypes (fastsar/ffbp_cpu.py:26-47, fastsar/exact.py:333-347); the
+ * stage-level entry points below take the same arguments as those kernels (fastsar/ffbp_cpu.cpp:244-471), with
+ * device pointers, interleaved complex data and a stream.
+ */
+#ifndef FSH_H
+#define FSH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSH_OK 0
+#define FSH_ENOMEM 1
+#define FSH_EINVAL 2
+#define FSH_ENONFINITE 3
+#define FSH_ERUNTIME 4
+
+/* ------------------------------------------------------------------------------------------ library */
+
+/* ABI version (major * 100 + minor). No reference counterpart. */
+int fsh_version(void);
+/* Device name and architecture into buf (n bytes); FSH_ERUNTIME without a device. Analog of the reference's
+ * backend probe (fastsar/api.py:66-90). */
+int fsh_device_info(char* buf, int n);
+/* The calling thread's last error message. */
+int fsh_last_error(char* buf, int n);
+
+/* Device memory for stage-level callers (tests, hosts without another allocator). */
+int fsh_malloc(void** ptr, size_t bytes);
+int fsh_free(void* ptr);
+int fsh_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int fsh_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int fsh_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int fsh_memset(void* dst, int value, size_t bytes);
+int fsh_synchronize(void);
+
+/* ------------------------------------------------------------------------------------------ stages (device pointers)
+ * stream: a hipStream_t, or NULL for the default stream. */
+
+/* Rotate each parent row by each child's phase ramp and decimate along frequency.
+ * x [Np, P, K] complex; c0, sl [Np, Cn, P] (cycles, cycles per sample); taps [L]; y [Np * Cn, P, Ko] complex:
+ *   y[n c, p, j] = sum_{r<L} taps[r] x[n, p, D j + r - pl] exp(2 pi i (c0 + (D j + r - pl - kc) sl)),
+ * x zero outside [0, K). Replaces rot_fir_k / rot_fir_k_cplx (fastsar/ffbp_cpu.cpp:244-255). */
+int fsh_rot_fir_k(const float* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
+                  int L, int pl, int D, int Ko, double kc, float* y, void* stream);
+
+/* Decimate along pulses: y [B, P, Ko] -> z [B, Po, Ko], z[b, i, j] = sum_{r<L} taps[r] y[b, D i + r - pl, j].
+ * Replaces fir_p (fastsar/ffbp_cpu.cpp:257-293). */
+int fsh_fir_p(const float* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float* z,
+              void* stream);
+
+/* Child rotations of a level computed on the device in float64: refs [Np, 3] parent centres, u [P, 3], r0 [P] the
+ * level's track, d [Cn, 3] child offsets -> c0, sl [Np, Cn, P] float32. Replaces the device-phase kernel and the
+ * host form (fastsar/ffbp_cuda.py:598-632, fastsar/ffbp_cpu.py:142-155). */
+int fsh_device_phases(const double* refs, const double* u, const double* r0, const double* d, int Np, int Cn, int P,
+                      double k0c, double k1, float* c0, float* sl, void* stream);
+
+/* Final geometry: fu [Pf, 3], fr0 [Pf], cen [B, 3] (device), e1, e2 [3] (host) -> ux, uy [B, Pf] and per tile
+ * tgeo [B, 3] = (ucx, ucy, rc). Replaces fastsar/ffbp_cpu.py:187-199 (fastsar/ffbp_cuda.py:870-886). */
+int fsh_final_geom(const double* fu, const double* fr0, const double* cen, int B, int Pf, const double* e1,
+                   const double* e2, double* ux, double* uy, double* tgeo, void* stream);
+
+/* Final stage: d [B, Pf, Qf] complex, T in {4, 8, 16, 32}, tile b written at pixel (tile_xy[b] - (ox, oy)) of the
+ * image out [nx, ny] complex, cropped. w0, gx, gyr [B, Pf] aperture weights or NULL.
+ * Replaces final_tiles (fastsar/ffbp_cpu.cpp:458-469), which writes [B, T, T]; tile_xy[b] = (b T, 0), nx = B T,
+ * ny = T, ox = oy = 0 reproduces that layout. */
+int fsh_final_tiles(const float* d, int B, int Pf, int Qf, int T, const double* ux, const double* uy,
+                    const double* tgeo, double spx, double spy, double a0, double a1, double fc2, const float* w0,
+                    const float* gx, const float* gyr, const int* tile_xy, int ox, int oy, int nx, int ny, float* out,
+                    void* stream);
+
+/* ------------------------------------------------------------------------------------------ FFBP former
+ * The plan (fastsar/ffbp2.py:192-250) and the collection arrays (fastsar/ffbp2.py:313-342) as computed on the
+ * host, uploaded once; fsh_ffbp_form then forms images. Replaces the backend factories
+ * make_ffbp_cpu / make_ffbp_cuda (fastsar/ffbp_cpu.py:158, fastsar/ffbp_cuda.py:889). */
+typedef struct fsh_level_desc {
+    int sx, sy, C, Dk, Dp, K, P, Ko, Po;
+    int Lk, plk, Lp, plp;
+    int nparents;                  /* parents at this level over the whole image (B) */
+    double kc, k0c, k1;            /* (K - 1) / 2; cycles per metre at band centre and per sample */
+    const float* taps_k;           /* [Lk] or NULL (Dk == 1) */
+    const float* taps_p;           /* [Lp] or NULL (Dp == 1) */
+    const float* c0;               /* [nparents, C, P] rotations (host levels), else NULL */
+    const float* sl;
+    const double* u;               /* [P, 3] track unit vectors (device-phase levels), else NULL */
+    const double* r0;              /* [P] */
+    const double* d;               /* [C, 3] child offsets */
+    const double* ref;             /* [nparents, 3] parent centres */
+} fsh_level_desc;
+
+typedef struct fsh_plan_desc {
+    int nlev, T, nx, ny, Nx, Ny, ox, oy, Pf, Qf, ntiles, P, K;
+    double spx, spy, a0, a1, fc2;
+    double e1[3], e2[3];
+    const double* fu;              /* [Pf, 3] final track unit vectors */
+    const double* fr0;             /* [Pf] */
+    const double* fcen;            /* [ntiles, 3] tile centres */
+    const int* tile_xy;            /* [ntiles, 2] first pixel of each tile on the padded grid */
+    const float* wp;               /* [P] pulse window or NULL */
+    const float* wk;               /* [K] sample window or NULL */
+    const float* w0;               /* [ntiles, Pf] aperture weights or NULL (fastsar/api.py:203-244) */
+    const float* gx;
+    const float* gyr;
+} fsh_plan_desc;
+
+int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, void** handle);
+/* S [P, K] complex (host or device), image [nx, ny] complex (host or device). bad_pulse: the first pulse with a
+ * non-finite sample when FSH_ENONFINITE is returned. */
+int fsh_ffbp_form(void* handle, const void* S, int S_on_device, void* image, int image_on_device, int* bad_pulse);
+/* First-level children per group (0: automatic, up to 8, halved while device memory lacks). */
+int fsh_ffbp_set_group(void* handle, int ng);
+/* The group used by the last call and the device bytes the former holds. */
+int fsh_ffbp_info(void* handle, int* ng_used, size_t* device_bytes);
+/* Per-stage device times (ms) of the last call: prepare, rot_fir_k, fir_p, device_phases, final_geom, final_tiles. */
+int fsh_ffbp_stage_ms(void* handle, float* ms, int n);
+int fsh_ffbp_destroy(void* handle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

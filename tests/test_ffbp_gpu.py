@@ -1,0 +1,133 @@
+"""HIP factorized backprojection against the float64 oracle: every stage on identical inputs, and whole images."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from fastsar_amd import _hip, plan as pl, sim
+from fastsar_amd.ffbp_hip import make_ffbp_hip
+
+pytestmark = pytest.mark.gpu
+
+
+def db(a, b):
+    return oracle.plain_db(a, b)
+
+
+def cplx(re, im):
+    return (np.asarray(re) + 1j * np.asarray(im)).astype(np.complex64)
+
+
+@pytest.mark.parametrize('D,L,Cn,Np', [(3, 23, 5, 2), (9, 67, 3, 1), (4, 31, 2, 3), (1, 1, 4, 1)])
+def test_rot_fir_k_stage(D, L, Cn, Np):
+    rng = np.random.default_rng(D * 100 + L)
+    P, K = 37, 301
+    x = (rng.standard_normal((Np, P, K)) + 1j * rng.standard_normal((Np, P, K))).astype(np.complex64)
+    c0 = rng.uniform(-0.5, 0.5, (Np, Cn, P)).astype(np.float32)
+    sl = rng.uniform(-0.8, 0.8, (Np, Cn, P)).astype(np.float32)
+    if D == 1:
+        fr, Ko, taps = None, K, np.ones(1, np.float32)
+    else:
+        fr, _ = pl.decimator_fir(K, D, 0.6)
+        Ko, taps = fr['n_out'], fr['kern'].astype(np.float32)
+    kc = (K - 1) / 2.0
+    yre, yim = oracle.rot_fir_k(x.real, x.imag, c0, sl, fr, D, Ko, kc)
+    dx, dc, ds, dt = (_hip.DeviceArray.from_host(a) for a in (x, c0, sl, taps))
+    dy = _hip.DeviceArray((Np * Cn, P, Ko), np.complex64)
+    _hip.check(_hip.lib().fsh_rot_fir_k(_hip.ptr(dx), Np, P, K, _hip.ptr(dc), _hip.ptr(ds), Cn, _hip.ptr(dt), len(taps),
+                                        0 if fr is None else fr['pl'], D, Ko, kc, _hip.ptr(dy), None))
+    got = dy.to_host()
+    e = db(got, yre + 1j * yim)
+    print(f'rot_fir_k D={D} L={L}: {e:.1f} dB')
+    assert e < -120, e
+
+
+@pytest.mark.parametrize('D,pf', [(2, 0.5), (11, 0.7), (5, 0.3)])
+def test_fir_p_stage(D, pf):
+    rng = np.random.default_rng(D)
+    B, P, Ko = 3, 613, 70
+    fr, _ = pl.decimator_fir(P, D, pf)
+    y = (rng.standard_normal((B, P, Ko)) + 1j * rng.standard_normal((B, P, Ko))).astype(np.complex64)
+    zre, zim = oracle.fir_p(y.real, y.imag, fr, D, fr['n_out'])
+    dy, dt = _hip.DeviceArray.from_host(y), _hip.DeviceArray.from_host(fr['kern'].astype(np.float32))
+    dz = _hip.DeviceArray((B, fr['n_out'], Ko), np.complex64)
+    _hip.check(_hip.lib().fsh_fir_p(_hip.ptr(dy), B, P, Ko, _hip.ptr(dt), fr['L'], fr['pl'], D, fr['n_out'], _hip.ptr(dz), None))
+    e = db(dz.to_host(), zre + 1j * zim)
+    print(f'fir_p D={D} L={fr["L"]}: {e:.1f} dB')
+    assert e < -125, e
+
+
+def _final_inputs(rng, B, Pf, Qf, T):
+    d = (rng.standard_normal((B, Pf, Qf)) + 1j * rng.standard_normal((B, Pf, Qf))).astype(np.complex64)
+    ux = rng.uniform(-0.1, 0.1, (B, Pf)) + 0.05
+    uy = rng.uniform(0.8, 0.9, (B, Pf))
+    geo = np.stack([rng.uniform(0, .1, B), rng.uniform(.8, .9, B), rng.uniform(9e3, 1.1e4, B)], 1)
+    return d, ux, uy, geo
+
+
+@pytest.mark.parametrize('T', [4, 8, 16, 32])
+def test_final_tiles_stage(T):
+    rng = np.random.default_rng(T)
+    B, Pf, Qf = 5, 23, 41
+    spx, spy = 0.5, 0.45
+    f0, df = 9.45e9, 3e6
+    a0, a1, fc2 = 2 * f0 / pl.C, 2 * df / pl.C, 2 * (f0 + (Qf - 1) / 2 * df) / pl.C
+    d, ux, uy, geo = _final_inputs(rng, B, Pf, Qf, T)
+    dlx = (np.arange(T) - (T - 1) / 2.0) * spx
+    dly = (np.arange(T) - (T - 1) / 2.0) * spy
+    ore, oim = oracle.final_tiles(d.real, d.imag, ux, uy, dlx, dly, a0, a1, geo[:, 0], geo[:, 1], geo[:, 2], fc2, T)
+    txy = np.stack([np.arange(B) * T, np.zeros(B)], 1).astype(np.int32)
+    dd, dux, duy, dgeo, dtxy = (_hip.DeviceArray.from_host(a) for a in (d, ux, uy, geo, txy))
+    out = _hip.DeviceArray((B * T, T), np.complex64)
+    _hip.check(_hip.lib().fsh_final_tiles(_hip.ptr(dd), B, Pf, Qf, T, _hip.ptr(dux), _hip.ptr(duy), _hip.ptr(dgeo), spx, spy,
+                                          a0, a1, fc2, None, None, None, _hip.ptr(dtxy), 0, 0, B * T, T, _hip.ptr(out), None))
+    e = db(out.to_host().reshape(B, T, T), ore + 1j * oim)
+    print(f'final_tiles T={T}: {e:.1f} dB')
+    assert e < -110, e
+
+
+def test_device_phases_stage():
+    rng = np.random.default_rng(4)
+    Np, Cn, P = 3, 6, 257
+    ant = np.stack([np.full(P, -8660.0), np.linspace(-200, 200, P), np.full(P, 5000.0)], 1)
+    r0 = np.linalg.norm(ant, axis=1)
+    u = ant / r0[:, None]
+    refs = rng.uniform(-300, 300, (Np, 3)) * [1, 1, 0]
+    d = rng.uniform(-40, 40, (Cn, 3)) * [1, 1, 0]
+    k0c, k1 = 64.0, 2.4e-4
+    c0, sl = pl.device_phases(u, r0, d, refs, k0c, k1)
+    args = [_hip.DeviceArray.from_host(a) for a in (refs, u, r0, d)]
+    dc, ds = _hip.DeviceArray((Np, Cn, P), np.float32), _hip.DeviceArray((Np, Cn, P), np.float32)
+    _hip.check(_hip.lib().fsh_device_phases(*[_hip.ptr(a) for a in args], Np, Cn, P, k0c, k1, _hip.ptr(dc), _hip.ptr(ds), None))
+    gc, gs = dc.to_host(), ds.to_host()
+    dphi = np.abs((gc - c0 + 0.5) % 1.0 - 0.5)
+    print(f'device_phases: max phase diff {dphi.max():.2e} cycles, slope {np.abs(gs - sl).max():.2e}')
+    assert dphi.max() < 1e-6 and np.allclose(gs, sl, rtol=1e-6, atol=0)
+
+
+@pytest.fixture(scope='module')
+def scene():
+    rng = np.random.default_rng(1)
+    col = sim.make_collect(res=0.5, scene=60.0, r0=5e3)
+    pos = np.stack([rng.uniform(-25, 25, 40), rng.uniform(-25, 25, 40), np.zeros(40)], 1)
+    amp = rng.standard_normal(40) + 1j * rng.standard_normal(40)
+    return col, sim.simulate_brute(col, pos, amp).astype(np.complex64)
+
+
+@pytest.mark.parametrize('nlev,T', [(2, 16), (3, 16), (3, 8), (2, 32), (4, 4)])
+def test_image_vs_oracle(scene, nlev, T):
+    col, S = scene
+    e1, e2 = (0.0, 1.0, 0.0), (1.0, 0.0, 0.0)
+    p = pl.make_plan(col, 128, 128, 0.5, 0.5, T=T, nlev=nlev, e1=e1, e2=e2)
+    coll = pl.collection_arrays(p, col.ant)
+    ref = oracle.ffbp_image(p, coll, S, 'oracle')
+    form = make_ffbp_hip(p, coll)
+    img = form(S)
+    e = db(img, ref)
+    print(f'image nlev={nlev} T={T}: hip vs float64 oracle {e:.1f} dB')
+    assert e < -100, e
+    # groups of first-level children: the same bits whatever the group
+    form.set_group(1)
+    img1 = form(S)
+    assert np.array_equal(img, img1)

@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark of the hot path: factorized backprojection of a 12,288 x 8,192 spotlight image from 16,384 pulses x
+8,192 samples (complex64) on one MI355X, with the error against a float64 exact backprojection on 16 seeded
+256 x 256 tiles, the dominant kernel's roofline fraction, and the reference's CPU former timed on a sample of the
+same image on the host's cores.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--T 8]
+
+A step is one image formation (ImageFormer call) with the phase history already resident in device memory and
+the image written to device memory; the window and the scan for NaN and inf are part of it. The data are
+synthetic: 2,000 point targets with Rayleigh amplitudes uniform over the 6 km x 4 km scene plus white clutter
+25 dB below the targets' power, X band (9.6 GHz, 300 MHz), 10 km slant range, 30 degrees grazing, 6 degree
+aperture. Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 's/image + Gpixel·pulses/s at 12288×8192, at ≤−60 dB error vs FP64 oracle'
+FP32_PEAK = 157.3e12          # MI355X published dense FP32 rate, FLOP/s
+HBM_PEAK = 8.0e12             # bytes/s
+
+
+def log(*a):
+    print('[bench]', *a, file=sys.stderr, flush=True)
+
+
+def workload(small=False):
+    """(col, grid, targets, clutter_db, seed): the 12,288 x 8,192 image from 16,384 x 8,192 samples, or (small) the
+    512 x 512 image from 512 x 1,024 samples."""
+    from fastsar_amd import sim
+    if small:
+        col = sim.spotlight(512, 1024, 3.0)
+        grid = dict(nx=512, ny=512, spx=0.5, spy=0.5, e1=(0.0, 1.0, 0.0), e2=(1.0, 0.0, 0.0))
+        return col, grid, 25, 256.0, 256.0, -30.0, 1
+    col = sim.spotlight(16384, 8192, 6.0)
+    sp = 6000.0 / 12288
+    grid = dict(nx=12288, ny=8192, spx=sp, spy=sp, e1=(0.0, 1.0, 0.0), e2=(1.0, 0.0, 0.0))
+    return col, grid, 2000, 6000.0, 4000.0, -25.0, 2
+
+
+def accuracy_points(grid, ntiles, side, seed):
+    rng = np.random.default_rng(seed)
+    nx, ny = grid['nx'], grid['ny']
+    side = min(side, nx, ny)
+    e1, e2 = np.asarray(grid['e1']), np.asarray(grid['e2'])
+    tiles, pts = [], []
+    for _ in range(ntiles):
+        i0, j0 = int(rng.integers(0, nx - side + 1)), int(rng.integers(0, ny - side + 1))
+        tiles.append((i0, j0))
+        I, J = np.meshgrid(np.arange(i0, i0 + side), np.arange(j0, j0 + side), indexing='ij')
+        p = ((I - nx / 2.0) * grid['spx'])[..., None] * e1 + ((J - ny / 2.0) * grid['spy'])[..., None] * e2
+        pts.append(p.reshape(-1, 3))
+    return tiles, np.concatenate(pts), side
+
+
+def cpu_baseline(col, grid, S_host, groups, T_cpu=16):
+    """The reference's CPU former (its own compiled kernels) on a sample of the first-level children: the window
+    pass over the whole history plus `groups` groups of 8 children, extrapolated to the whole image."""
+    import oracle
+    from fastsar_amd import plan as pl
+    from fastsar_amd.api import _window
+    threads = os.cpu_count() or 1
+    os.environ.setdefault('OMP_PLACES', 'cores')
+    os.environ.setdefault('OMP_PROC_BIND', 'close')
+    L = oracle.ref_lib()
+    threads = int(L.ffbp_cpu_threads())
+    p = pl.make_plan(col, grid['nx'], grid['ny'], grid['spx'], grid['spy'], T=T_cpu, nlev=3, e1=grid['e1'], e2=grid['e2'])
+    coll = pl.collection_arrays(p, col.ant)
+    former = oracle.RefCpuFormer(p, coll)
+    G = p['levels'][0]['C']
+    wp, wk = _window(col.Np, col.K)
+    former(S_host[:, :], children=[G - 1], ng=1)                   # warm: one child, untimed
+    t0 = time.perf_counter()
+    Sw = former.window(S_host, wp, wk)
+    t_win = time.perf_counter() - t0
+    kids = list(range(min(G, 8 * groups)))
+    t0 = time.perf_counter()
+    former(Sw, children=kids, ng=8)
+    t_kids = time.perf_counter() - t0
+    t_full = t_win + t_kids * G / len(kids)
+    pairs = grid['nx'] * grid['ny'] * col.Np
+    return dict(value=round(pairs / t_full / 1e9, 3), unit='Gpixel·pulses/s', cores=threads, kind='reference',
+                sample=(f'reference CPU former (fastsar/ffbp_cpu.cpp built from its sources, x86-64-{L.isa}, OpenMP '
+                        f'{threads} threads), T={T_cpu}: window pass over the whole history ({t_win:.2f} s) plus '
+                        f'{len(kids)} of {G} first-level children ({t_kids:.2f} s), extrapolated to '
+                        f'{t_full:.2f} s per image'),
+                s_per_image=round(t_full, 3))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--T', type=int, default=8)
+    ap.add_argument('--tiles', type=int, default=16, help='accuracy tiles (256 x 256) against float64 backprojection')
+    ap.add_argument('--upsample', type=int, default=64, help='oversampling of the float64 checker profiles')
+    ap.add_argument('--cpu-groups', type=int, default=2, help='groups of 8 first-level children timed on the CPU')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--small', action='store_true', help='the 512 x 512 scene instead (quick check)')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py runs on one GPU: --gpus {args.gpus} is not supported (the image-formation path needs no '
+              'collective; see DESIGN.md for how it would shard)', file=sys.stderr)
+        sys.exit(2)
+
+    from fastsar_amd import ImageFormer, _hip, sim
+    from fastsar_amd.exact import simulate_device, bp_points_f64_device
+    from fastsar_amd.metrics import rel_db, stage_flops
+    from fastsar_amd.api import _window
+
+    dev = _hip.device_info()
+    log('device', dev)
+    col, grid, ntg, ext1, ext2, cl_db, seed = workload(args.small)
+    rng = np.random.default_rng(seed)
+    pos, amp = sim.point_scene(rng, ntg, ext1, ext2, grid['e1'], grid['e2'])
+    t0 = time.perf_counter()
+    dS = simulate_device(col, pos, amp, clutter_sigma=sim.clutter_sigma(amp, cl_db), seed=seed)
+    log(f'phase history {col.Np} x {col.K} generated on the device in {time.perf_counter() - t0:.2f} s')
+
+    t0 = time.perf_counter()
+    former = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T)
+    plan = former._plan
+    log(f'former built in {time.perf_counter() - t0:.2f} s: T={former.T}, levels={former.levels}, '
+        f'splits={plan["split"]}, final {plan["final"]["P"]} x {plan["final"]["K"]}, predicted {former.predicted_error_db:.1f} dB')
+    for l, lv in enumerate(plan['levels']):
+        log(f'  level {l}: {lv["sx"]}x{lv["sy"]} children, Dk={lv["Dk"]} Dp={lv["Dp"]}, [{lv["P"]} x {lv["K"]}] -> '
+            f'[{lv["Po"]} x {lv["Ko"]}], Lk={lv["fir_k"]["L"] if lv["fir_k"] else 1}, Lp={lv["fir_p"]["L"] if lv["fir_p"] else 1}')
+    img = _hip.DeviceArray((grid['nx'], grid['ny']), np.complex64)
+    for i in range(args.warmup):
+        former(dS, out=img)
+    _hip.check(_hip.lib().fsh_synchronize())
+    log(f'warm: group of {former._form.info()["group"]} first-level children')
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        former(dS, out=img)
+    _hip.check(_hip.lib().fsh_synchronize())
+    dt = (time.perf_counter() - t0) / args.steps
+    pairs = grid['nx'] * grid['ny'] * col.Np
+    log(f'{dt * 1e3:.1f} ms per image, {pairs / dt / 1e9:.1f} Gpixel.pulses/s')
+
+    # per-launch device times (events on the former's stream) of one more call
+    former._form.set_profile(True)
+    former(dS, out=img)
+    prof = former._form.profile()
+    former._form.set_profile(False)
+    agg = {}
+    for st, lvl, ms in prof:
+        a = agg.setdefault((st, lvl), [0.0, 0])
+        a[0] += ms
+        a[1] += 1
+    flops = stage_flops(plan)
+    stages = {f'{k[0]}[{k[1]}]': dict(ms=round(v[0], 3), launches=v[1],
+                                      tflops=round(flops[k] / (v[0] * 1e-3) / 1e12, 2) if k in flops else None)
+              for k, v in sorted(agg.items(), key=lambda kv: -kv[1][0])}
+    for k, v in stages.items():
+        log(f'  {k}: {v}')
+    dom = max((k for k in agg if k in flops), key=lambda k: agg[k][0])
+    dom_ms = agg[dom][0] / agg[dom][1]
+    dom_flop = flops[dom] / agg[dom][1]
+    achieved = dom_flop / (dom_ms * 1e-3)
+    roofline = dict(bound='mfma', achieved=round(achieved / 1e12, 3), peak=FP32_PEAK / 1e12, unit='TFLOP/s',
+                    frac=round(achieved / FP32_PEAK, 4), traffic=None, kernel=f'{dom[0]} level {dom[1]}',
+                    flop_per_launch=dom_flop, ms_per_launch=round(dom_ms, 4))
+
+    # error against float64 exact backprojection on seeded tiles
+    t0 = time.perf_counter()
+    tiles, pts, side = accuracy_points(grid, args.tiles, 256, seed=7)
+    nfft = 1 << int(np.ceil(np.log2(args.upsample * col.K)))
+    wp, wk = _window(col.Np, col.K)
+    ref_vals = bp_points_f64_device(dS, col.ant, np.linalg.norm(col.ant, axis=1), col.fmin, col.df, pts, nfft, False, wp, wk,
+                                    chunk=max(1, (1 << 30) // (nfft * 16)))
+    himg = img.to_host()
+    got = np.concatenate([himg[i0:i0 + side, j0:j0 + side].ravel() for i0, j0 in tiles])
+    err = rel_db(got, ref_vals)
+    log(f'error vs float64 exact backprojection ({len(tiles)} tiles of {side}^2, linear reads at {args.upsample}x): '
+        f'{err:.2f} dB ({time.perf_counter() - t0:.1f} s)')
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            import oracle
+            if oracle.have_ref():
+                t0 = time.perf_counter()
+                cpu = cpu_baseline(col, grid, dS.to_host(), args.cpu_groups)
+                log(f'cpu baseline: {cpu} ({time.perf_counter() - t0:.1f} s)')
+        except Exception as e:          # the baseline is reported, not required
+            log(f'cpu baseline skipped: {type(e).__name__}: {e}')
+
+    out = dict(metric=METRIC, value=round(pairs / dt / 1e9, 2), unit='Gpixel·pulses/s', n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=round(dt * 1e3, 2), higher_is_better=True, scaling='weak',
+               vs_baseline=None, dtype='f32', data='synthetic',
+               config=dict(workload=(f'{"512x512 from 512x1024" if args.small else "12288x8192 image from 16384 pulses x 8192 samples"}'
+                                     f' complex64, factorized backprojection, T={former.T}, {former.levels} levels'),
+                           s_per_image=round(dt, 4), error_db=round(err, 2), error_tiles=f'{len(tiles)} x {side}^2',
+                           predicted_error_db=round(former.predicted_error_db, 2), parallelism='single GPU'),
+               roofline=roofline, cpu_baseline=cpu, stages=stages)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

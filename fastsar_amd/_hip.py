@@ -67,6 +67,8 @@ def lib():
         'fsh_ffbp_set_group': ([vp, c_int], c_int),
         'fsh_ffbp_info': ([vp, P_i32, ctypes.POINTER(c_size_t)], c_int),
         'fsh_ffbp_stage_ms': ([vp, P_f32, c_int], c_int),
+        'fsh_ffbp_set_profile': ([vp, c_int], c_int),
+        'fsh_ffbp_profile': ([vp, P_i32, P_i32, P_f32, c_int, P_i32], c_int),
         'fsh_ffbp_destroy': ([vp], c_int),
     }
     for name, (args, res) in sig.items():

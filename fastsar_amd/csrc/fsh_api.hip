@@ -133,6 +133,9 @@ struct Former {
     bool profile = false;
     float stage_ms[ST_N] = {0};
     hipEvent_t ev[2] = {nullptr, nullptr};
+    struct Rec { int stage, level; float ms; };
+    std::vector<Rec> recs;         // one per launch of the last profiled call
+    int cur_level = -1;
 
     ~Former() { release_group(); release_all(); }
 
@@ -209,6 +212,7 @@ struct Former {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
         stage_ms[stage] += ms;
+        recs.push_back(Rec{stage, cur_level, ms});
     }
 
     int run_group(int g0, int gn)
@@ -222,6 +226,7 @@ struct Former {
             const Level& L = lv[l];
             const fsh_level_desc& h = L.h;
             const int ch = (l == 0) ? gn : h.C;
+            cur_level = (int)l;
             const int yb = (curbuf == 0) ? 1 : 0;
             const float* c0;
             const float* sl;
@@ -260,6 +265,7 @@ struct Former {
             parents *= ch;
         }
         const long long tiles = parents, t0 = first_parent;
+        cur_level = (int)lv.size();
         mark_begin();
         if ((st = launch_final_geom(fu, fr0, fcen + 3 * t0, (int)tiles, p.Pf, e1, e2, ux, uy, tgeo, stream))) return st;
         mark_end(ST_GEOM);
@@ -438,6 +444,26 @@ int fsh_ffbp_info(void* handle, int* ng_used, size_t* device_bytes)
     return OK;
 }
 
+int fsh_ffbp_set_profile(void* handle, int on)
+{
+    if (!handle) { set_error("ffbp_set_profile: null handle"); return E_INVALID; }
+    static_cast<Former*>(handle)->profile = on != 0;
+    return OK;
+}
+
+int fsh_ffbp_profile(void* handle, int* stage, int* level, float* ms, int cap, int* count)
+{
+    if (!handle || !count) { set_error("ffbp_profile: null argument"); return E_INVALID; }
+    Former* f = static_cast<Former*>(handle);
+    *count = (int)f->recs.size();
+    for (int i = 0; i < cap && i < (int)f->recs.size(); ++i) {
+        if (stage) stage[i] = f->recs[i].stage;
+        if (level) level[i] = f->recs[i].level;
+        if (ms) ms[i] = f->recs[i].ms;
+    }
+    return OK;
+}
+
 int fsh_ffbp_stage_ms(void* handle, float* ms, int n)
 {
     if (!handle || !ms) { set_error("ffbp_stage_ms: null argument"); return E_INVALID; }
@@ -453,6 +479,8 @@ int fsh_ffbp_form(void* handle, const void* S, int S_on_device, void* image, int
     const fsh_plan_desc& p = f->p;
     int st;
     for (int i = 0; i < ST_N; ++i) f->stage_ms[i] = 0.f;
+    f->recs.clear();
+    f->cur_level = -1;
     const float2* src = (const float2*)S;
     if (!S_on_device) {
         if (!f->hist && (st = dev_alloc((void**)&f->hist, (size_t)p.P * p.K * sizeof(float2)))) return st;

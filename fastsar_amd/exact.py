@@ -1,5 +1,54 @@
-"""Exact backprojection onto the form_image grid (fastsar/exact.py) and onto points (fastsar/bp.py)."""
+"""Exact backprojection onto the form_image grid (ExactFormer, fastsar/exact.py) and onto points (backproject,
+fastsar/bp.py), on the 'hip' backend.
+
+    former = ExactFormer(ant, fmin, df, K, nx, ny, spx, spy, e1, e2)      # backend='auto'
+    img = former(S)
+
+The grid is form_image's, about the scene reference point or `center`. The host plans, once per former and in
+float64: the tile size (the largest whose third-order range expansion error h^4 / (2 r^3), as a phase at f_max,
+stays under 3e-4 rad), the tile centres, and the window of range bins each pulse can reach. A call range-compresses
+each chunk of pulses on the device (window, centred zero padding to nfft = 2^ceil(log2(upsample K)), inverse FFT),
+keeps the reachable bins and backprojects them: per tile and pulse the centre terms in float64, per pixel the
+expansion in float32 and a cubic (or linear) read.
+
+backproject(..., backend='hip') forms onto arbitrary points with exact float64 ranges and phases and complex128
+profiles: the float64 path of fastsar/bp.py (its 'cpu' backend), on the device.
+"""
+import ctypes
+import warnings
+
 import numpy as np
+
+from . import _hip
+from .plan import C
+
+TILES = ((32, 32), (16, 16), (8, 16), (8, 8))     # candidate tiles, largest first (fastsar/exact.py:366-368)
+PHASE_LIMIT = 3e-4                                 # rad (fastsar/exact.py:369)
+
+
+class ExactDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ('P', 'K', 'nx', 'ny', 'tx', 'ty', 'ntx', 'nty', 'nfft', 'W', 'chunk', 'cubic')] + \
+               [(n, ctypes.c_double) for n in ('spx', 'spy', 'inv_dr', 'kcyc')] + \
+               [('e1', ctypes.c_double * 3), ('e2', ctypes.c_double * 3),
+                ('ant', _hip.P_f64), ('ref', _hip.P_f64), ('cen', _hip.P_f64), ('blo', ctypes.POINTER(ctypes.c_longlong)),
+                ('wp', _hip.P_f32), ('wk', _hip.P_f32)]
+
+
+def _bind():
+    L = _hip.lib()
+    if not getattr(L, '_exact_bound', False):
+        vp, ci = ctypes.c_void_p, ctypes.c_int
+        L.fsh_exact_create.argtypes = [ctypes.POINTER(ExactDesc), ctypes.POINTER(vp)]
+        L.fsh_exact_form.argtypes = [vp, vp, ci, vp, ci, _hip.P_i32]
+        L.fsh_exact_destroy.argtypes = [vp]
+        L.fsh_bp_points_f64.argtypes = [vp, ci, ci, vp, vp, vp, vp, vp, ctypes.c_long, ctypes.c_double, ctypes.c_double, ci,
+                                        ci, vp, ci]
+        L.fsh_sim_points.argtypes = [vp, vp, ci, ci, ctypes.c_double, ctypes.c_double, vp, vp, ci, vp, vp]
+        L.fsh_add_noise.argtypes = [vp, ctypes.c_size_t, ctypes.c_double, ctypes.c_ulonglong, vp]
+        for n in ('fsh_exact_create', 'fsh_exact_form', 'fsh_exact_destroy', 'fsh_bp_points_f64', 'fsh_sim_points', 'fsh_add_noise'):
+            getattr(L, n).restype = ci
+        L._exact_bound = True
+    return L
 
 
 def plane_points(nx, ny, spx, spy, e1=(1.0, 0.0, 0.0), e2=(0.0, 1.0, 0.0), height=None):
@@ -15,10 +64,258 @@ def plane_points(nx, ny, spx, spy, e1=(1.0, 0.0, 0.0), e2=(0.0, 1.0, 0.0), heigh
     return p
 
 
+def _integer(v, name, least):
+    try:
+        ok = not isinstance(v, (bool, np.bool_)) and np.isscalar(v) and np.isfinite(v) and int(v) == v and v >= least
+    except (TypeError, ValueError):
+        ok = False
+    if not ok:
+        raise ValueError(f'{name} must be an integer of at least {least}, got {v!r}')
+    return int(v)
+
+
+def _number(v):
+    try:
+        return float(v)
+    except (TypeError, ValueError):
+        return float('nan')
+
+
+def _distance_to_rectangle(ant, x0, e1, e2, a0, a1, b0, b1):
+    """Per pulse the distance from ant [P, 3] to the rectangle x0 + a e1 + b e2 (a0 <= a <= a1, b0 <= b <= b1): the
+    antenna projected onto the plane and clamped into the rectangle, the exact minimum of the convex distance.
+    (fastsar/exact.py:389-396)"""
+    v = ant - x0
+    a = np.clip(v @ e1, a0, a1)
+    b = np.clip(v @ e2, b0, b1)
+    return np.linalg.norm(v - a[:, None] * e1 - b[:, None] * e2, axis=1)
+
+
 class ExactFormer:
-    def __init__(self, *a, **k):
-        raise NotImplementedError('ExactFormer on hip: not built yet')
+    """Exact backprojection onto the grid of form_image, set up once for a collection geometry. former(S) ->
+    complex64 image [nx, ny]. interp 'cubic' (default) or 'linear'; upsample the range oversampling (default 8);
+    chunk the pulses range compressed at a time; ref [P] the reference ranges (default |ant|); center the grid's
+    centre (default the origin). former.tile and former.predicted_error_db as in fastsar/exact.py:404-540. A former
+    must not be called from two threads at once."""
+
+    def __init__(self, ant, fmin, df, K, nx, ny, spx, spy, e1=(1.0, 0.0, 0.0), e2=(0.0, 1.0, 0.0), *, backend='auto',
+                 window=True, interp='cubic', upsample=None, ref=None, center=None, chunk=1024, profiles='auto'):
+        from .api import _backend, _check_positions, _check_grid, _window
+        if interp not in ('cubic', 'linear'):
+            raise ValueError(f"interp must be 'cubic' or 'linear', got {interp!r}")
+        if profiles not in ('auto', 'float32'):
+            raise ValueError(f"profiles must be 'auto' or 'float32' on the hip backend, got {profiles!r}")
+        self.profiles = 'float32'
+        self.ant = _check_positions(ant)
+        self.P = len(self.ant)
+        self.K = _integer(K, 'K (frequency samples per pulse)', 2)
+        if not (np.isfinite(_number(fmin)) and np.isfinite(_number(df)) and _number(df) > 0):
+            raise ValueError(f'fmin must be a finite frequency and df a positive frequency spacing in Hz, got {fmin} and {df}')
+        fmin, df = float(fmin), float(df)
+        nx, ny, e1, e2 = _check_grid(nx, ny, spx, spy, e1, e2)
+        self.nx, self.ny, self.spx, self.spy = int(nx), int(ny), float(spx), float(spy)
+        self.e1, self.e2 = e1, e2
+        if center is None:
+            self.center = np.zeros(3)
+        else:
+            try:
+                self.center = np.asarray(center, np.float64)
+            except (TypeError, ValueError):
+                raise ValueError(f'center must be a point (a finite 3-vector) in meters, got {center!r}') from None
+            if self.center.shape != (3,) or not np.isfinite(self.center).all():
+                raise ValueError(f'center must be a point (a finite 3-vector) in meters, got {center!r}')
+        if ref is None:
+            self.ref = np.linalg.norm(self.ant, axis=1)
+        else:
+            try:
+                self.ref = np.asarray(ref, np.float64)
+            except (TypeError, ValueError):
+                raise ValueError('ref must be a range per pulse in meters') from None
+            if self.ref.shape != (self.P,):
+                raise ValueError(f'ref must hold one range per pulse, shape ({self.P},), got {self.ref.shape}')
+            if not np.isfinite(self.ref).all():
+                raise ValueError(f'ref has non-finite values (NaN or inf), the first in pulse {int(np.argmax(~np.isfinite(self.ref)))}')
+        self.cubic = interp == 'cubic'
+        self.upsample = 8 if upsample is None else _integer(upsample, 'upsample', 1)
+        self.chunk = _integer(chunk, 'chunk', 1)
+        self.backend = _backend(backend)
+        self.fmin, self.df, self.window = fmin, df, window
+        self.nfft = 1 << int(np.ceil(np.log2(self.upsample * self.K)))
+        self.inv_dr = 2.0 * df * self.nfft / C
+        self.kcyc = 2.0 * (fmin + (self.K // 2) * df) / C
+        self.wp, self.wk = _window(self.P, self.K) if window else (None, None)
+        self._choose_tile()
+        self.ntx, self.nty = -(-nx // self.tx), -(-ny // self.ty)
+        x0 = self.center + (-nx / 2.0) * spx * self.e1 + (-ny / 2.0) * spy * self.e2
+        bi = (np.arange(self.ntx) * self.tx + 0.5 * (self.tx - 1))[:, None, None]
+        bj = (np.arange(self.nty) * self.ty + 0.5 * (self.ty - 1))[None, :, None]
+        self.cen = np.ascontiguousarray((x0 + bi * spx * self.e1 + bj * spy * self.e2).reshape(-1, 3))
+        # the bins each pulse can reach: the grid with a tile of margin, nearest by projection and clamping, farthest
+        # at a corner; then the expansion's error and the interpolation's taps (fastsar/exact.py:478-507)
+        mx, my = self.tx * spx, self.ty * spy
+        a0, a1, b0, b1 = -mx, (nx - 1) * spx + mx, -my, (ny - 1) * spy + my
+        near = _distance_to_rectangle(self.ant, x0, self.e1, self.e2, a0, a1, b0, b1)
+        corners = np.array([x0 + a * self.e1 + b * self.e2 for a in (a0, a1) for b in (b0, b1)])
+        far = np.linalg.norm(corners[None] - self.ant[:, None], axis=2).max(1)
+        slack = 4 + int(np.ceil(self._expansion_m * self.inv_dr))
+        blo = np.floor((near - self.ref) * self.inv_dr).astype(np.int64) + self.nfft // 2 - slack
+        bhi = np.ceil((far - self.ref) * self.inv_dr).astype(np.int64) + self.nfft // 2 + slack
+        self.W = int((bhi - blo).max()) + 4
+        near0 = _distance_to_rectangle(self.ant, x0, self.e1, self.e2, 0.0, (nx - 1) * spx, 0.0, (ny - 1) * spy)
+        far0 = np.linalg.norm(np.array([x0 + a * self.e1 + b * self.e2 for a in (0.0, (nx - 1) * spx)
+                                         for b in (0.0, (ny - 1) * spy)])[None] - self.ant[:, None], axis=2).max(1)
+        need = int((np.ceil(far0 * self.inv_dr) - np.floor(near0 * self.inv_dr)).max()) + 2 * slack + 4
+        if need > self.nfft:
+            raise ValueError(f'the grid spans up to {float((far0 - near0).max()):.1f} m of range from one pulse, more than '
+                             f'this collection resolves without ambiguity (c / (2 df) = {C / (2 * df):.1f} m less the '
+                             f'interpolation margin: the grid needs {need} of the {self.nfft} range bins at '
+                             f'upsample={self.upsample}); form a smaller grid (or several) or use samples spaced more '
+                             'finely in frequency')
+        self.blo = np.ascontiguousarray(blo, np.int64)
+        self.lo = (blo - self.nfft // 2).astype(np.int32)
+        self._h = None
+
+    def _choose_tile(self):
+        """The largest tile whose predicted phase error stays below PHASE_LIMIT. (fastsar/exact.py:511-540)"""
+        nx, ny, spx, spy = self.nx, self.ny, self.spx, self.spy
+        x0 = self.center + (-nx / 2.0) * spx * self.e1 + (-ny / 2.0) * spy * self.e2
+        rmin = float(_distance_to_rectangle(self.ant, x0, self.e1, self.e2, 0.0, (nx - 1) * spx, 0.0, (ny - 1) * spy).min())
+        kmax = 4 * np.pi * (self.fmin + (self.K - 1) * self.df) / C
+        best = None
+        for tx, ty in TILES:
+            h = 0.5 * np.hypot((tx - 1) * spx, (ty - 1) * spy)
+            r = rmin - h
+            if r < 4 * h:
+                continue
+            err = h ** 4 / (2 * r ** 3)
+            best = (tx, ty, err, kmax * err)
+            if kmax * err <= PHASE_LIMIT:
+                break
+        if best is None:
+            raise ValueError(f'the grid comes within {rmin:.1f} m of the antenna, too close for the tiled range expansion '
+                             f'of ExactFormer; use backproject on plane_points of the grid')
+        self.tx, self.ty, self._expansion_m, phase = best
+        self.tile = (self.tx, self.ty)
+        self.predicted_error_db = float(20 * np.log10(max(phase, 1e-30)))
+        if phase > PHASE_LIMIT:
+            warnings.warn(f'ExactFormer: at {rmin:.0f} m from the antenna, even {self.tx} by {self.ty} pixel tiles leave a '
+                          f'predicted phase error of {phase:.1e} rad ({self.predicted_error_db:.1f} dB) from the range '
+                          'expansion; backproject on plane_points of the grid is exact at any range',
+                          UserWarning, stacklevel=3)
+
+    def _handle(self):
+        if self._h is None:
+            L = _bind()
+            d = ExactDesc()
+            d.P, d.K, d.nx, d.ny, d.tx, d.ty = self.P, self.K, self.nx, self.ny, self.tx, self.ty
+            d.ntx, d.nty, d.nfft, d.W, d.chunk, d.cubic = self.ntx, self.nty, self.nfft, self.W, min(self.chunk, self.P), int(self.cubic)
+            d.spx, d.spy, d.inv_dr, d.kcyc = self.spx, self.spy, self.inv_dr, self.kcyc
+            for k in range(3):
+                d.e1[k], d.e2[k] = self.e1[k], self.e2[k]
+            keep = [np.ascontiguousarray(self.ant), np.ascontiguousarray(self.ref), self.cen, self.blo]
+            d.ant = keep[0].ctypes.data_as(_hip.P_f64)
+            d.ref = keep[1].ctypes.data_as(_hip.P_f64)
+            d.cen = keep[2].ctypes.data_as(_hip.P_f64)
+            d.blo = keep[3].ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
+            if self.wp is not None:
+                keep += [np.ascontiguousarray(self.wp, np.float32), np.ascontiguousarray(self.wk, np.float32)]
+                d.wp, d.wk = keep[-2].ctypes.data_as(_hip.P_f32), keep[-1].ctypes.data_as(_hip.P_f32)
+            h = ctypes.c_void_p()
+            _hip.check(L.fsh_exact_create(ctypes.byref(d), ctypes.byref(h)), 'exact_create')
+            self._h = h
+        return self._h
+
+    def memory(self):
+        """dict(backend, needed, available, full_speed, parts) in bytes. (fastsar/exact.py:736-761)"""
+        ch = min(self.chunk, self.P)
+        parts = dict(history=self.P * self.K * 8, padded=ch * self.nfft * 8, fft=ch * self.nfft * 8,
+                     window=ch * self.W * 8, image=self.nx * self.ny * 8, geometry=self.cen.nbytes + self.P * 40)
+        need = float(sum(parts.values()))
+        avail = float(_hip.device_info()['mem'])
+        return dict(backend=self.backend, needed=need, available=avail, full_speed=need <= avail, parts=parts)
+
+    def __call__(self, S, out=None):
+        """S [P, K] complex phase history (numpy or a DeviceArray) -> complex64 image [nx, ny] (or into out, a
+        DeviceArray). NaN and inf are found on the device and raise ValueError naming the first pulse."""
+        from .api import _check_history
+        on_dev = isinstance(S, _hip.DeviceArray)
+        if not on_dev:
+            S = _check_history(np.asarray(S), self.P, finite=False)
+            if S.shape[1] != self.K:
+                raise ValueError(f'phase history must have {self.K} samples per pulse, got {S.shape[1]}')
+            S = np.ascontiguousarray(S, np.complex64)
+        elif S.shape != (self.P, self.K) or S.dtype != np.complex64:
+            raise ValueError(f'phase history must be complex64 {(self.P, self.K)}')
+        h = self._handle()
+        img_dev = isinstance(out, _hip.DeviceArray)
+        img = out if img_dev else np.empty((self.nx, self.ny), np.complex64)
+        bad = ctypes.c_int(-1)
+        _hip.check(_bind().fsh_exact_form(h, _hip.ptr(S), int(on_dev), _hip.ptr(img), int(img_dev), ctypes.byref(bad)),
+                   'exact_form')
+        return img
+
+    def __del__(self):
+        try:
+            if self._h is not None and _hip._lib is not None:
+                _hip._lib.fsh_exact_destroy(self._h)
+        except Exception:
+            pass
 
 
-def backproject(*a, **k):
-    raise NotImplementedError('backproject on hip: not built yet')
+def backproject(S, ant, fmin, df, points, *, ref=None, backend='auto', upsample=8, window=True, chunk=256,
+                interp='linear'):
+    """Complex image at points [..., 3] (complex128, shape [...]) by exact float64 backprojection: profiles
+    oversampled `upsample` times (nfft a power of two) in complex128, read linearly (or cubic) at the exact float64
+    range of every point and pulse. ref [P]: the reference ranges (default |ant|). Monostatic.
+    (fastsar/bp.py:303-348, its float64 'cpu' path)"""
+    from .api import _backend, _check_history, _check_positions, _window
+    _backend(backend)
+    S = _check_history(np.asarray(S))
+    P, K = S.shape
+    tx = _check_positions(ant, 'antenna positions', P)
+    points = np.asarray(points, np.float64)
+    if points.ndim < 1 or points.shape[-1] != 3 or points.size == 0:
+        raise ValueError(f'points must be [..., 3] with at least one point, got shape {points.shape}')
+    ref = np.linalg.norm(tx, axis=1) if ref is None else np.asarray(ref, np.float64)
+    if ref.shape != (P,):
+        raise ValueError(f'ref must hold one range per pulse, shape ({P},), got {ref.shape}')
+    shape = points.shape[:-1]
+    nfft = 1 << int(np.ceil(np.log2(upsample * K)))
+    wp, wk = _window(P, K) if window else (None, None)
+    dS = _hip.DeviceArray.from_host(np.ascontiguousarray(S, np.complex64))
+    return bp_points_f64_device(dS, tx, ref, fmin, df, points.reshape(-1, 3), nfft, interp == 'cubic', wp, wk, chunk).reshape(shape)
+
+
+def bp_points_f64_device(dS, ant, ref, fmin, df, pts, nfft, cubic=False, wp=None, wk=None, chunk=256):
+    """fsh_bp_points_f64 on a device history dS [P, K] complex64 -> complex128 [N] on the host."""
+    L = _bind()
+    P, K = dS.shape
+    dant = _hip.DeviceArray.from_host(np.ascontiguousarray(ant, np.float64))
+    dref = _hip.DeviceArray.from_host(np.ascontiguousarray(ref, np.float64))
+    dpts = _hip.DeviceArray.from_host(np.ascontiguousarray(pts, np.float64))
+    out = _hip.DeviceArray((len(pts),), np.complex128).zero()
+    dwp = None if wp is None else _hip.DeviceArray.from_host(np.ascontiguousarray(wp, np.float32))
+    dwk = None if wk is None else _hip.DeviceArray.from_host(np.ascontiguousarray(wk, np.float32))
+    _hip.check(L.fsh_bp_points_f64(_hip.ptr(dS), P, K, _hip.ptr(dwp), _hip.ptr(dwk), _hip.ptr(dant), _hip.ptr(dref),
+                                   _hip.ptr(dpts), len(pts), float(fmin), float(df), int(nfft), int(bool(cubic)), _hip.ptr(out),
+                                   int(chunk)), 'bp_points_f64')
+    return out.to_host()
+
+
+def simulate_device(col, pos, amp, ref=None, clutter_sigma=None, seed=0):
+    """The phase history of a point scene on the device (fsh_sim_points: float64 ranges and phases, float64 sums,
+    stored complex64) plus optional white clutter: a DeviceArray [P, K] complex64."""
+    L = _bind()
+    P, K = col.ant.shape[0], col.K
+    ant = np.ascontiguousarray(col.ant, np.float64)
+    ref = np.linalg.norm(ant, axis=1) if ref is None else np.asarray(ref, np.float64)
+    S = _hip.DeviceArray((P, K), np.complex64).zero()
+    dant, dref = _hip.DeviceArray.from_host(ant), _hip.DeviceArray.from_host(np.ascontiguousarray(ref))
+    dpos = _hip.DeviceArray.from_host(np.ascontiguousarray(pos, np.float64))
+    damp = _hip.DeviceArray.from_host(np.ascontiguousarray(np.asarray(amp, np.complex128)))
+    _hip.check(L.fsh_sim_points(_hip.ptr(dant), _hip.ptr(dref), P, K, float(col.fmin), float(col.df), _hip.ptr(dpos),
+                                _hip.ptr(damp), len(pos), _hip.ptr(S), None), 'sim_points')
+    if clutter_sigma:
+        _hip.check(L.fsh_add_noise(_hip.ptr(S), P * K, float(clutter_sigma), int(seed), None), 'add_noise')
+    _hip.check(L.fsh_synchronize(), 'synchronize')
+    return S

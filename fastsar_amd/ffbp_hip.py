@@ -114,6 +114,18 @@ class HipFFBP:
         _hip.check(_hip.lib().fsh_ffbp_info(self.h, ctypes.byref(ng), ctypes.byref(nb)), 'ffbp_info')
         return dict(group=ng.value, device_bytes=nb.value)
 
+    def set_profile(self, on):
+        """Per-launch device timing (events on the former's stream, each launch waited for) of the next calls."""
+        _hip.check(_hip.lib().fsh_ffbp_set_profile(self.h, int(bool(on))), 'ffbp_set_profile')
+
+    def profile(self):
+        """[(stage, level, ms)] per launch of the last profiled call; level == nlev for the final stage."""
+        n = ctypes.c_int()
+        _hip.check(_hip.lib().fsh_ffbp_profile(self.h, None, None, None, 0, ctypes.byref(n)), 'ffbp_profile')
+        st, lv, ms = (ctypes.c_int * n.value)(), (ctypes.c_int * n.value)(), (ctypes.c_float * n.value)()
+        _hip.check(_hip.lib().fsh_ffbp_profile(self.h, st, lv, ms, n.value, ctypes.byref(n)), 'ffbp_profile')
+        return [(STAGES[st[i]], lv[i], float(ms[i])) for i in range(n.value)]
+
     def stage_ms(self):
         ms = (ctypes.c_float * len(STAGES))()
         _hip.check(_hip.lib().fsh_ffbp_stage_ms(self.h, ms, len(STAGES)), 'ffbp_stage_ms')

@@ -126,7 +126,48 @@ int fsh_ffbp_set_group(void* handle, int ng);
 int fsh_ffbp_info(void* handle, int* ng_used, size_t* device_bytes);
 /* Per-stage device times (ms) of the last call: prepare, rot_fir_k, fir_p, device_phases, final_geom, final_tiles. */
 int fsh_ffbp_stage_ms(void* handle, float* ms, int n);
+/* Profiling (no reference counterpart; the analog of ffbp_cuda.PROFILE, fastsar/ffbp_cuda.py:635-652): when on,
+ * every launch is bracketed by events on the former's stream and waited for; fsh_ffbp_profile then gives, per
+ * launch of the last call, its stage (the order above), its level (nlev for the final stage) and its device ms. */
+int fsh_ffbp_set_profile(void* handle, int on);
+int fsh_ffbp_profile(void* handle, int* stage, int* level, float* ms, int cap, int* count);
 int fsh_ffbp_destroy(void* handle);
+
+/* ------------------------------------------------------------------------------------------ exact former
+ * ExactFormer's geometry as planned on the host (fastsar/exact.py:415-540): tiles of tx x ty pixels with centres
+ * cen, profiles of nfft bins, of which W reachable ones are kept per pulse from bin blo[p] (zero delay at nfft / 2).
+ * Replaces ExactFormer._form_cuda / _form_cpu and the bp_tiles kernel (fastsar/exact.py:566-695, 240-243). */
+typedef struct fsh_exact_desc {
+    int P, K, nx, ny, tx, ty, ntx, nty, nfft, W, chunk, cubic;
+    double spx, spy, inv_dr, kcyc;
+    double e1[3], e2[3];
+    const double* ant;             /* [P, 3] */
+    const double* ref;             /* [P] */
+    const double* cen;             /* [ntx nty, 3] */
+    const long long* blo;          /* [P] */
+    const float* wp;               /* [P] or NULL */
+    const float* wk;               /* [K] or NULL */
+} fsh_exact_desc;
+
+int fsh_exact_create(const fsh_exact_desc* desc, void** handle);
+int fsh_exact_form(void* handle, const void* S, int S_on_device, void* image, int image_on_device, int* bad_pulse);
+int fsh_exact_destroy(void* handle);
+
+/* Exact backprojection onto points in float64 (device pointers): S [P, K] complex64 windowed by wp, wk (or NULL),
+ * profiles of nfft bins (a power of two) in complex128, exact float64 range and phase per point and pulse, linear
+ * (cubic = 0) or cubic Lagrange reads; out [N] complex128 accumulates. The float64 path of
+ * fastsar/bp.py:99-170 (backend 'cpu'), on the device. */
+int fsh_bp_points_f64(const float* S, int P, int K, const float* wp, const float* wk, const double* ant,
+                      const double* ref, const double* pts, long N, double fmin, double df, int nfft, int cubic,
+                      double* out, int chunk);
+
+/* ------------------------------------------------------------------------------------------ synthetic data
+ * Test and benchmark scenes (no reference counterpart on the device; the host sum is fastsar/sim.py:90-105). */
+/* S [P, K] complex += sum_n amp_n exp(-j 4 pi f_k (|pos_n - ant_p| - ref_p) / c), f_k = fmin + k df; all device. */
+int fsh_sim_points(const double* ant, const double* ref, int P, int K, double fmin, double df, const double* pos,
+                   const double* amp, int N, float* S, void* stream);
+/* S [n] complex += sigma / sqrt(2) (g1 + j g2), standard normals from a counter-based hash of (seed, index). */
+int fsh_add_noise(float* S, size_t n, double sigma, unsigned long long seed, void* stream);
 
 #ifdef __cplusplus
 }

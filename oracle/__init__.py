@@ -337,6 +337,86 @@ def ffbp_image(plan, coll, S, stages='ref', device_phases=None):
     return full[ox:ox + nx, oy:oy + ny]
 
 
+class RefCpuFormer:
+    """The reference's CPU former (fastsar/ffbp_cpu.py:158-291) over its compiled kernels: level-0 children in
+    groups of up to 8, the windowed complex64 history read in place (rot_fir_k_cplx), each child carried through the
+    remaining levels (levels >= 2 with float64 NumPy phases) and final_tiles, the blocks written into the image.
+    form(S, children=...) forms only the listed level-0 children (the CPU baseline's bounded sample)."""
+
+    def __init__(self, plan, coll):
+        from fastsar_amd import plan as planner
+        self.plan, self.coll, self.planner = plan, coll, planner
+        if plan['T'] not in (16, 32):
+            raise ValueError("the reference's CPU former takes T = 16 or 32")
+        lv = plan['levels']
+        self.G = lv[0]['C']
+
+    def window(self, S, wp, wk):
+        """One complex64 copy windowed in row blocks (fastsar/api.py:512-519)."""
+        out = np.empty(S.shape, np.complex64)
+        for p0 in range(0, S.shape[0], 4096):
+            sl = slice(p0, min(S.shape[0], p0 + 4096))
+            np.multiply(S[sl], (wp[sl, None] * wk[None, :]).astype(np.float32), out=out[sl])
+        return out
+
+    def __call__(self, S, children=None, ng=8):
+        plan, coll, planner = self.plan, self.coll, self.planner
+        L = ref_lib()
+        levels, T, nlev = plan['levels'], plan['T'], len(plan['levels'])
+        fin = plan['final']
+        a0, a1, fc2 = 2.0 * fin['f0'] / C, 2.0 * fin['df'] / C, 2.0 * fin['fc'] / C
+        dlx = np.ascontiguousarray((np.arange(T) - (T - 1) / 2.0) * plan['spx'])
+        dly = np.ascontiguousarray((np.arange(T) - (T - 1) / 2.0) * plan['spy'])
+        fu, fr0, fcen = coll['final']['u'], coll['final']['r0'], fin['cen']
+        sx0, sy0 = levels[0]['sx'], levels[0]['sy']
+        G = self.G
+        mx, my = plan['Nx'] // sx0, plan['Ny'] // sy0
+        shape_g = [s for lv in levels[1:] for s in (lv['sx'], lv['sy'])] + [T, T]
+        perm_g = [2 * i for i in range(nlev - 1)] + [2 * (nlev - 1)] + [2 * i + 1 for i in range(nlev - 1)] + [2 * (nlev - 1) + 1]
+        S = np.ascontiguousarray(S, np.complex64)
+        lv0 = levels[0]
+        if lv0['Dk'] <= 1:
+            raise ValueError('the in-place complex64 first level needs range decimation at level 0')
+        full = np.empty((plan['Nx'], plan['Ny']), np.complex64)
+        todo = list(range(G)) if children is None else list(children)
+
+        def kids(a, b, c0, sl, lv):
+            y = ref_rot_fir_k(a, b, c0, sl, lv['fir_k'], lv['Dk'], lv['Ko'], (lv['K'] - 1) / 2.0)
+            return ref_fir_p(y[0], y[1], lv['fir_p'], lv['Dp'], lv['Po']) if lv['Dp'] > 1 else y
+
+        for g0 in range(0, len(todo), ng):
+            gs = todo[g0:g0 + ng]
+            c0 = np.ascontiguousarray(coll['levels'][0]['c0'][0, gs][None])
+            sl = np.ascontiguousarray(coll['levels'][0]['slope'][0, gs][None])
+            fk = lv0['fir_k']
+            yre = np.empty((len(gs), lv0['P'], lv0['Ko']), np.float32)
+            yim = np.empty((len(gs), lv0['P'], lv0['Ko']), np.float32)
+            L.rot_fir_k_cplx(S.view(np.float32).reshape(S.shape[0], -1), 1, lv0['P'], lv0['K'], c0, sl, len(gs),
+                             np.ascontiguousarray(fk['kern'], np.float32), fk['L'], fk['pl'], lv0['Dk'], lv0['Ko'],
+                             (lv0['K'] - 1) / 2.0, yre, yim)
+            A, B = (ref_fir_p(yre, yim, lv0['fir_p'], lv0['Dp'], lv0['Po']) if lv0['Dp'] > 1 else (yre, yim))
+            for k, g in enumerate(gs):
+                a, b = A[k][None], B[k][None]
+                for i in range(1, nlev):
+                    lv, cl = levels[i], coll['levels'][i]
+                    if i < planner.HOST_LEVELS:
+                        c0, sl = cl['c0'][g][None], cl['slope'][g][None]
+                    else:
+                        refs = lv['ref'].reshape(G, a.shape[0], 3)[g]
+                        k0c, k1 = planner.level_phase_constants(lv)
+                        c0, sl = planner.device_phases(cl['u'], cl['r0'], lv['d'], refs, k0c, k1)
+                    a, b = kids(a, b, c0, sl, lv)
+                cen = fcen.reshape(G, -1, 3)[g]
+                ux, uy, ucx, ucy, rc = final_geometry(fu, fr0, cen, plan['e1'], plan['e2'])
+                re, im = ref_final_tiles(a, b, ux, uy, dlx, dly, a0, a1, ucx, ucy, rc, fc2, T)
+                blk = (re.reshape(shape_g).transpose(perm_g).reshape(mx, my)
+                       + 1j * im.reshape(shape_g).transpose(perm_g).reshape(mx, my))
+                x, y = g // sy0, g % sy0
+                full[x * mx:(x + 1) * mx, y * my:(y + 1) * my] = blk
+        ox, oy, nx, ny = plan['ox'], plan['oy'], plan['nx'], plan['ny']
+        return full[ox:ox + nx, oy:oy + ny]
+
+
 def exact_tiles(win, lo, ant, ref, cen, ntx, nty, tx, ty, nx, ny, spx, spy, e1, e2, inv_dr, kcyc, cubic):
     """float64 ExactFormer tiles on cropped profiles win [P, W] complex -> image [nx, ny] complex128."""
     P, W = win.shape

@@ -101,6 +101,7 @@ def main():
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--T', type=int, default=8)
+    ap.add_argument('--levels', type=int, default=4, help='levels of the plan (the former API default is 3, more when 3 cannot split)')
     ap.add_argument('--tiles', type=int, default=16, help='accuracy tiles (256 x 256) against float64 backprojection')
     ap.add_argument('--upsample', type=int, default=64, help='oversampling of the float64 checker profiles')
     ap.add_argument('--cpu-groups', type=int, default=2, help='groups of 8 first-level children timed on the CPU')
@@ -127,7 +128,7 @@ def main():
     log(f'phase history {col.Np} x {col.K} generated on the device in {time.perf_counter() - t0:.2f} s')
 
     t0 = time.perf_counter()
-    former = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T)
+    former = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T, levels=args.levels or None)
     plan = former._plan
     log(f'former built in {time.perf_counter() - t0:.2f} s: T={former.T}, levels={former.levels}, '
         f'splits={plan["split"]}, final {plan["final"]["P"]} x {plan["final"]["K"]}, predicted {former.predicted_error_db:.1f} dB')

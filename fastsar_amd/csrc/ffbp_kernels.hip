@@ -15,6 +15,9 @@
 #include "fsh_common.h"
 #include "fsh_kernels.h"
 
+#include <cstdlib>
+#include <cstring>
+
 namespace fsh {
 
 // ------------------------------------------------------------------------------------------------ prepare
@@ -104,6 +107,211 @@ __global__ __launch_bounds__(256) void k_rot_fir_k(const float2* __restrict__ x,
         }
         __syncthreads();
     }
+}
+
+// ------------------------------------------------------------------------------------------------ rot_fir_k on MFMA
+//
+// The same stage as a matrix product per parent row p. With k_j = D j - pl and the child's ramp factored at k_j,
+//   y_c[j] = rho_c(j) sum_r x[k_j + r] g_c[r],   g_c[r] = h[r] exp(2 pi i sl_c r),
+//   rho_c(j) = exp(2 pi i (c0_c + (k_j - kc) sl_c)),
+// i.e. Y = X G with X [Ko x L] the row's sliding windows (shared by all children) and G [L x C] complex taps per
+// child. As a real product: X_real [Ko x 2L] with (Re, Im) interleaved along k (the complex64 row itself), G_real
+// [2L x 2C] with columns (Re y_c, Im y_c) interleaved: rows (Re x, Im x) of column Re y_c hold (Re g, -Im g), of
+// column Im y_c (Im g, Re g). Operands are split into bfloat16 pairs (v = hi + lo) and multiplied in three passes
+// (hi hi + hi lo + lo hi) on v_mfma_f32_32x32x16_bf16 with float32 accumulation: float32-class accuracy.
+//
+// Layout of v_mfma_f32_32x32x16_bf16 (lane l, q = l / 32): A row l % 32, k 8q .. 8q + 7; B column l % 32, k 8q ..
+// 8q + 7; result element i at row 8 (i / 4) + 4 q + i % 4, column l % 32.
+//
+// grid (ceil(P / 4), M-chunks, parents), block 256 = 4 waves, wave w forms pulse 4 blockIdx.x + w. Each wave stages
+// its row window as packed bfloat16 (Re, Im) words, hi and lo planes, skewed by one word every 32 against LDS bank
+// conflicts; per N-tile (16 children) it builds G in registers by recurrences seeded in float64, then for each
+// M-tile (32 outputs) runs KS k-steps of three MFMAs, rotates the result by rho (seeded in float64 per tile, stepped
+// in float32 within it) and writes it through an LDS transpose so that each half-wave stores 32 consecutive
+// outputs of one child.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx16_t __attribute__((ext_vector_type(16)));
+typedef unsigned int uint4_t __attribute__((ext_vector_type(4)));
+
+constexpr int MF_STAGE = 16 * 66;        // per-wave transpose buffer (floats): 16 children x 32 outputs x (Re, Im), padded
+
+__device__ __forceinline__ unsigned int pack_bf16(__bf16 lo, __bf16 hi)
+{
+    return (unsigned int)__builtin_bit_cast(unsigned short, lo) | ((unsigned int)__builtin_bit_cast(unsigned short, hi) << 16);
+}
+
+__device__ __forceinline__ void split_bf16(float v, __bf16& hi, __bf16& lo)
+{
+    hi = (__bf16)v;
+    lo = (__bf16)(v - (float)hi);
+}
+
+__device__ __forceinline__ int skew(int i) { return i + (i >> 5); }
+
+template <int KS>
+__global__ __launch_bounds__(256) void k_rot_fir_k_mfma(const float2* __restrict__ x, int P, int K, const float* __restrict__ c0,
+                                                         const float* __restrict__ sl, int Cn, const float* __restrict__ taps,
+                                                         int L, int pl, int D, int Ko, double kc, int MC, int WIN,
+                                                         float2* __restrict__ y)
+{
+    extern __shared__ unsigned int lds_u[];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 31, q = lane >> 5;
+    const int p = blockIdx.x * 4 + wv;
+    const int n = blockIdx.z;
+    const int MT = (Ko + 31) / 32;
+    const int m0 = blockIdx.y * MC, m1 = min(m0 + MC, MT);
+    const int WS = skew(WIN) + 1;
+    unsigned int* XH = lds_u + wv * (2 * WS + MF_STAGE);
+    unsigned int* XL = XH + WS;
+    float* stg = reinterpret_cast<float*>(XL + WS);
+    float* tp = reinterpret_cast<float*>(lds_u + 4 * (2 * WS + MF_STAGE));
+    for (int r = threadIdx.x; r < 8 * KS; r += 256) tp[r] = r < L ? taps[r] : 0.f;
+    const bool pok = p < P;
+    const int ks = D * 32 * m0 - pl;
+    const float2* xrow = x + ((size_t)n * P + (pok ? p : 0)) * K;
+    for (int i = lane; i < WIN; i += 64) {
+        const int k = ks + i;
+        const float2 v = (pok && k >= 0 && k < K) ? xrow[k] : make_float2(0.f, 0.f);
+        __bf16 rh, rl, ih, il;
+        split_bf16(v.x, rh, rl);
+        split_bf16(v.y, ih, il);
+        XH[skew(i)] = pack_bf16(rh, ih);
+        XL[skew(i)] = pack_bf16(rl, il);
+    }
+    __syncthreads();
+    const int NT = (2 * Cn + 31) / 32;
+    for (int nt = 0; nt < NT; ++nt) {
+        const int ncol = nt * 32 + col;
+        const int c = ncol >> 1, t = ncol & 1;
+        const bool cok = pok && c < Cn;
+        double a = 0.0, b = 0.0;
+        if (cok) {
+            a = (double)c0[((size_t)n * Cn + c) * P + p];
+            b = (double)sl[((size_t)n * Cn + c) * P + p];
+        }
+        bf16x8_t Gh[KS], Gl[KS];
+        {
+            float wr, wi, e1r, e1i, e8r, e8i;
+            cis_cyc(b * (double)(4 * q), wr, wi);
+            cis_cyc(b, e1r, e1i);
+            cis_cyc(8.0 * b, e8r, e8i);
+            #pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                float vr = wr, vi = wi;
+                #pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float h = cok ? tp[8 * s + 4 * q + e] : 0.f;
+                    const float gr = h * vr, gi = h * vi;
+                    const float u0 = t ? gi : gr, u1 = t ? gr : -gi;
+                    __bf16 h0, l0, h1, l1;
+                    split_bf16(u0, h0, l0);
+                    split_bf16(u1, h1, l1);
+                    Gh[s][2 * e] = h0; Gh[s][2 * e + 1] = h1;
+                    Gl[s][2 * e] = l0; Gl[s][2 * e + 1] = l1;
+                    const float nr = fmaf(vr, e1r, -(vi * e1i));
+                    vi = fmaf(vr, e1i, vi * e1r);
+                    vr = nr;
+                }
+                const float nr = fmaf(wr, e8r, -(wi * e8i));
+                wi = fmaf(wr, e8i, wi * e8r);
+                wr = nr;
+            }
+        }
+        float sDr, sDi, s8r, s8i;
+        cis_cyc(b * (double)D, sDr, sDi);
+        cis_cyc(b * (double)(8 * D), s8r, s8i);
+        for (int mt = m0; mt < m1; ++mt) {
+            floatx16_t acc = {};
+            const int base = D * ((mt - m0) * 32 + col) + 4 * q;
+            #pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int i0 = base + 8 * s;
+                uint4_t wh, wl;
+                wh.x = XH[skew(i0)]; wh.y = XH[skew(i0 + 1)]; wh.z = XH[skew(i0 + 2)]; wh.w = XH[skew(i0 + 3)];
+                wl.x = XL[skew(i0)]; wl.y = XL[skew(i0 + 1)]; wl.z = XL[skew(i0 + 2)]; wl.w = XL[skew(i0 + 3)];
+                const bf16x8_t Ah = __builtin_bit_cast(bf16x8_t, wh), Al = __builtin_bit_cast(bf16x8_t, wl);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gh[s], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gl[s], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Gh[s], acc, 0, 0, 0);
+            }
+            // rho at rows 4q, 4q + 1, ... of this tile: seeded at row 4q, stepped by D samples, 8 rows apart by 8 D
+            float rr, ri;
+            cis_cyc(fma((double)(D * (mt * 32 + 4 * q) - pl) - kc, b, a), rr, ri);
+            #pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                float er = rr, ei = ri;
+                #pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int i = 4 * g + e;
+                    const float mine = acc[i];
+                    const float other = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, mine), 0xB1, 0xF, 0xF, false));
+                    const float yr = t ? other : mine, yi = t ? mine : other;
+                    const float outr = fmaf(yr, er, -(yi * ei)), outi = fmaf(yr, ei, yi * er);
+                    stg[(col >> 1) * 66 + 2 * (8 * g + 4 * q + e) + t] = t ? outi : outr;
+                    const float nr = fmaf(er, sDr, -(ei * sDi));
+                    ei = fmaf(er, sDi, ei * sDr);
+                    er = nr;
+                }
+                const float nr = fmaf(rr, s8r, -(ri * s8i));
+                ri = fmaf(rr, s8i, ri * s8r);
+                rr = nr;
+            }
+            __syncthreads();
+            const int j = mt * 32 + col;
+            #pragma unroll
+            for (int it = 0; it < 8; ++it) {
+                const int cl = q + 2 * it, cc = nt * 16 + cl;
+                if (pok && cc < Cn && j < Ko) {
+                    const float2 v = *reinterpret_cast<const float2*>(stg + cl * 66 + 2 * col);
+                    y[(((size_t)n * Cn + cc) * P + p) * Ko + j] = v;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+int launch_rot_fir_k_mfma(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
+                          int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s)
+{
+    const int KS = (L + 7) / 8;
+    if (KS < 1 || KS > 12) return E_INVALID;
+    const int MT = (Ko + 31) / 32;
+    const int win_max = 1600;
+    int MC = std::max(1, (win_max - 8 * KS + D) / (32 * D));
+    MC = std::min(MC, MT);
+    const int WIN = D * (32 * MC - 1) + 8 * KS;
+    const int WS = WIN + (WIN >> 5) + 1;
+    const size_t lds = (size_t)4 * (4 * (2 * WS + MF_STAGE)) + 4 * 8 * KS;
+    if (lds > 160 * 1024) return E_INVALID;
+    const int gx = (P + 3) / 4, gy = (MT + MC - 1) / MC;
+    for (int n0 = 0; n0 < Np; n0 += 65535) {
+        const int nn = std::min(65535, Np - n0);
+        dim3 grid(gx, gy, nn);
+        const float2* xs = x + (size_t)n0 * P * K;
+        const float* c0s = c0 + (size_t)n0 * Cn * P;
+        const float* sls = sl + (size_t)n0 * Cn * P;
+        float2* ys = y + (size_t)n0 * Cn * P * Ko;
+#define FSH_MF(KK) hipLaunchKernelGGL(k_rot_fir_k_mfma<KK>, grid, dim3(256), lds, s, xs, P, K, c0s, sls, Cn, taps, L, pl, D, Ko, kc, MC, WIN, ys)
+        switch (KS) {
+            case 1: FSH_MF(1); break;
+            case 2: FSH_MF(2); break;
+            case 3: FSH_MF(3); break;
+            case 4: FSH_MF(4); break;
+            case 5: FSH_MF(5); break;
+            case 6: FSH_MF(6); break;
+            case 7: FSH_MF(7); break;
+            case 8: FSH_MF(8); break;
+            case 9: FSH_MF(9); break;
+            case 10: FSH_MF(10); break;
+            case 11: FSH_MF(11); break;
+            default: FSH_MF(12); break;
+        }
+#undef FSH_MF
+        FSH_TRY(hipGetLastError());
+    }
+    return OK;
 }
 
 // ------------------------------------------------------------------------------------------------ fir_p
@@ -378,6 +586,14 @@ int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, con
     if (Np < 1 || P < 1 || K < 1 || Cn < 1 || L < 1 || D < 1 || Ko < 1 || pl < 0) {
         set_error("rot_fir_k: invalid shape");
         return E_INVALID;
+    }
+    // the matrix-core form where its taps fit (L <= 96) and the axis is decimated; FSH_ROT_KERNEL=valu|mfma forces one
+    const char* force = getenv("FSH_ROT_KERNEL");
+    const bool want_valu = force && strcmp(force, "valu") == 0;
+    const bool want_mfma = force && strcmp(force, "mfma") == 0;
+    if (!want_valu && (D > 1 || want_mfma) && L <= 96) {
+        const int st = launch_rot_fir_k_mfma(x, Np, P, K, c0, sl, Cn, taps, L, pl, D, Ko, kc, y, s);
+        if (st != E_INVALID) return st;
     }
     const int PB = 8;
     const int budget = std::min(lds_limit(), 65536);

@@ -276,7 +276,7 @@ int launch_rot_fir_k_mfma(const float2* x, int Np, int P, int K, const float* c0
                           int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s)
 {
     const int KS = (L + 7) / 8;
-    if (KS < 1 || KS > 12) return E_INVALID;
+    if (KS < 1 || KS > 16) return E_INVALID;
     const int MT = (Ko + 31) / 32;
     const int win_max = 1600;
     int MC = std::max(1, (win_max - 8 * KS + D) / (32 * D));
@@ -306,7 +306,11 @@ int launch_rot_fir_k_mfma(const float2* x, int Np, int P, int K, const float* c0
             case 9: FSH_MF(9); break;
             case 10: FSH_MF(10); break;
             case 11: FSH_MF(11); break;
-            default: FSH_MF(12); break;
+            case 12: FSH_MF(12); break;
+            case 13: FSH_MF(13); break;
+            case 14: FSH_MF(14); break;
+            case 15: FSH_MF(15); break;
+            default: FSH_MF(16); break;
         }
 #undef FSH_MF
         FSH_TRY(hipGetLastError());
@@ -348,6 +352,50 @@ __global__ __launch_bounds__(256) void k_fir_p(const float2* __restrict__ y, int
             ai = fmaf(t, v.y, ai);
         }
         z[((size_t)b * Po + i) * Ko + j] = make_float2(ar, ai);
+    }
+}
+
+// Streaming form: one thread per column (b, j) and run of IC outputs; the input rows are read once, in order, and
+// each is added into the NW = ceil(L / D) outputs whose taps cover it, held as a sliding window of accumulators. Row
+// p = D m - pl + s (period m, phase s < D) feeds output i with tap D (m - i) + s; at the end of period m output
+// m - NW + 1 is complete, is stored, and the window shifts. Taps are zero padded to D NW in LDS. Consecutive
+// threads take consecutive columns, so every row load is coalesced. grid (ceil(B Ko / 256), ceil(Po / IC)).
+template <int NW>
+__global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__ y, int B, int P, int Ko,
+                                                       const float* __restrict__ taps, int L, int pl, int D, int Po, int IC,
+                                                       float2* __restrict__ z)
+{
+    extern __shared__ float tpz[];                   // [D * NW], zero beyond L
+    for (int r = threadIdx.x; r < D * NW; r += 256) tpz[r] = r < L ? taps[r] : 0.f;
+    __syncthreads();
+    const long col = (long)blockIdx.x * 256 + threadIdx.x;
+    if (col >= (long)B * Ko) return;
+    const int b = (int)(col / Ko), j = (int)(col - (long)b * Ko);
+    const int i0 = blockIdx.y * IC, i1 = min(Po, i0 + IC);
+    const float2* yb = y + (size_t)b * P * Ko + j;
+    float2* zb = z + (size_t)b * Po * Ko + j;
+    float2 acc[NW];
+    #pragma unroll
+    for (int s = 0; s < NW; ++s) acc[s] = make_float2(0.f, 0.f);
+    // slot s holds output m - NW + 1 + s during period m; output i takes periods i .. i + NW - 1, so the periods run
+    // from i0 to i1 + NW - 2 and output m - NW + 1 is stored at the end of period m when it lies in [i0, i1)
+    for (int m = i0; m < i1 + NW - 1; ++m) {
+        const int pbase = D * m - pl;
+        for (int ph = 0; ph < D; ++ph) {
+            const int p = pbase + ph;
+            const float2 v = (p >= 0 && p < P) ? yb[(size_t)p * Ko] : make_float2(0.f, 0.f);
+            #pragma unroll
+            for (int s = 0; s < NW; ++s) {
+                const float h = tpz[D * (NW - 1 - s) + ph];
+                acc[s].x = fmaf(h, v.x, acc[s].x);
+                acc[s].y = fmaf(h, v.y, acc[s].y);
+            }
+        }
+        const int done = m - NW + 1;
+        if (done >= i0 && done < i1) zb[(size_t)done * Ko] = acc[0];
+        #pragma unroll
+        for (int s = 0; s < NW - 1; ++s) acc[s] = acc[s + 1];
+        acc[NW - 1] = make_float2(0.f, 0.f);
     }
 }
 
@@ -557,6 +605,106 @@ __global__ __launch_bounds__(256) void k_final_tiles(const float2* __restrict__ 
     }
 }
 
+// Register-blocked final stage: one thread forms a BI x BI block of one tile (BI = min(T, 8)), with no LDS. For
+// subaperture p and sample q (frequency factor f_q = a0 + a1 q) the table entries along the pixel index are
+// geometric: A_i = d exp(-2 pi i ux f_q dlx_i) = a_{i0} r^(i - i0) with r = exp(-2 pi i ux spx f_q), and likewise
+// B_j. Per p the block's first entry and the ratio are seeded in float64 at q = 0 and advanced over q by their
+// own ratios (one complex multiply each); per q the block's A and B rows take BI - 1 multiplies each, then BI^2
+// complex multiply-adds. Each pixel sums its (p, q) terms in order. W: aperture weights present.
+template <int T, bool W>
+__global__ __launch_bounds__(256) void k_final_tiles_reg(const float2* __restrict__ data, int B, int Pf, int Qf,
+                                                          const double* __restrict__ ux, const double* __restrict__ uy,
+                                                          const double* __restrict__ tgeo, double spx, double spy, double a0,
+                                                          double a1, double fc2, const float* __restrict__ w0,
+                                                          const float* __restrict__ gx, const float* __restrict__ gyr,
+                                                          const int* __restrict__ tile_xy, int ox, int oy, int nx, int ny,
+                                                          float2* __restrict__ out)
+{
+    constexpr int BI = T < 8 ? T : 8;
+    constexpr int NB = T / BI;                     // blocks per tile side
+    const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+    const long b = gid / (NB * NB);
+    if (b >= B) return;
+    const int blk = (int)(gid - b * NB * NB), bx = blk / NB, by = blk - bx * NB;
+    const int i0 = bx * BI, j0 = by * BI;
+    const double hc = 0.5 * (T - 1);
+    const double dx0 = (i0 - hc) * spx, dy0 = (j0 - hc) * spy;
+    float2 acc[BI][BI];
+    #pragma unroll
+    for (int a = 0; a < BI; ++a)
+        #pragma unroll
+        for (int c = 0; c < BI; ++c) acc[a][c] = make_float2(0.f, 0.f);
+    const float2* dp = data + (size_t)b * Pf * Qf;
+    for (int p = 0; p < Pf; ++p) {
+        const double gxp = ux[(size_t)b * Pf + p], gyp = uy[(size_t)b * Pf + p];
+        float2 ra, rb, ba, bb, sra, srb, sba, sbb;
+        cis_cyc(-(gxp * spx) * a0, ra.x, ra.y);         // ratio along i at q = 0
+        cis_cyc(-(gxp * spx) * a1, sra.x, sra.y);       // its step over q
+        cis_cyc(-(gxp * dx0) * a0, ba.x, ba.y);         // entry at i0, q = 0
+        cis_cyc(-(gxp * dx0) * a1, sba.x, sba.y);
+        cis_cyc(-(gyp * spy) * a0, rb.x, rb.y);
+        cis_cyc(-(gyp * spy) * a1, srb.x, srb.y);
+        cis_cyc(-(gyp * dy0) * a0, bb.x, bb.y);
+        cis_cyc(-(gyp * dy0) * a1, sbb.x, sbb.y);
+        float wa[BI], wb[BI];
+        if (W) {
+            const size_t bp = (size_t)b * Pf + p;
+            #pragma unroll
+            for (int k = 0; k < BI; ++k) {
+                wa[k] = w0[bp] + gx[bp] * (float)(dx0 + k * spx);
+                wb[k] = 1.f + gyr[bp] * (float)(dy0 + k * spy);
+            }
+        }
+        const float2* dq = dp + (size_t)p * Qf;
+        for (int q = 0; q < Qf; ++q) {
+            float2 av[BI], bv[BI];
+            av[0] = cmul(dq[q], ba);
+            bv[0] = bb;
+            #pragma unroll
+            for (int k = 1; k < BI; ++k) {
+                av[k] = cmul(av[k - 1], ra);
+                bv[k] = cmul(bv[k - 1], rb);
+            }
+            if (W) {
+                #pragma unroll
+                for (int k = 0; k < BI; ++k) {
+                    av[k].x *= wa[k]; av[k].y *= wa[k];
+                    bv[k].x *= wb[k]; bv[k].y *= wb[k];
+                }
+            }
+            #pragma unroll
+            for (int a = 0; a < BI; ++a)
+                #pragma unroll
+                for (int c = 0; c < BI; ++c) {
+                    acc[a][c].x = fmaf(av[a].x, bv[c].x, acc[a][c].x);
+                    acc[a][c].x = fmaf(-av[a].y, bv[c].y, acc[a][c].x);
+                    acc[a][c].y = fmaf(av[a].x, bv[c].y, acc[a][c].y);
+                    acc[a][c].y = fmaf(av[a].y, bv[c].x, acc[a][c].y);
+                }
+            ra = cmul(ra, sra);
+            ba = cmul(ba, sba);
+            rb = cmul(rb, srb);
+            bb = cmul(bb, sbb);
+        }
+    }
+    const double cx = tgeo[3 * b], cy = tgeo[3 * b + 1], f = fc2 / (2.0 * tgeo[3 * b + 2]);
+    const int X0 = tile_xy[2 * b] - ox + i0, Y0 = tile_xy[2 * b + 1] - oy + j0;
+    #pragma unroll
+    for (int a = 0; a < BI; ++a) {
+        const double dx = dx0 + a * spx;
+        #pragma unroll
+        for (int c = 0; c < BI; ++c) {
+            const int X = X0 + a, Y = Y0 + c;
+            if (X < 0 || X >= nx || Y < 0 || Y >= ny) continue;
+            const double dy = dy0 + c * spy;
+            const double los = cx * dx + cy * dy;
+            float cs, sn;
+            cis_cyc((dx * dx + dy * dy - los * los) * f, cs, sn);
+            out[(size_t)X * ny + Y] = cmul(acc[a][c], make_float2(cs, sn));
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------------ launchers
 
 static int lds_limit()
@@ -587,11 +735,11 @@ int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, con
         set_error("rot_fir_k: invalid shape");
         return E_INVALID;
     }
-    // the matrix-core form where its taps fit (L <= 96) and the axis is decimated; FSH_ROT_KERNEL=valu|mfma forces one
+    // the matrix-core form where its taps fit (L <= 128) and the axis is decimated; FSH_ROT_KERNEL=valu|mfma forces one
     const char* force = getenv("FSH_ROT_KERNEL");
     const bool want_valu = force && strcmp(force, "valu") == 0;
     const bool want_mfma = force && strcmp(force, "mfma") == 0;
-    if (!want_valu && (D > 1 || want_mfma) && L <= 96) {
+    if (!want_valu && (D > 1 || want_mfma) && L <= 128) {
         const int st = launch_rot_fir_k_mfma(x, Np, P, K, c0, sl, Cn, taps, L, pl, D, Ko, kc, y, s);
         if (st != E_INVALID) return st;
     }
@@ -621,6 +769,35 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
     if (B < 1 || P < 1 || Ko < 1 || L < 1 || D < 1 || Po < 1 || pl < 0) {
         set_error("fir_p: invalid shape");
         return E_INVALID;
+    }
+    const int NW = (L + D - 1) / D;
+    const char* force = getenv("FSH_FIRP_KERNEL");
+    if (NW <= 16 && !(force && strcmp(force, "lds") == 0)) {
+        const int IC = std::min(Po, 128);
+        dim3 grid((unsigned)(((long)B * Ko + 255) / 256), (Po + IC - 1) / IC);
+        const size_t lds = (size_t)4 * D * NW;
+#define FSH_FP(NN) hipLaunchKernelGGL(k_fir_p_stream<NN>, grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z)
+        switch (NW) {
+            case 1: FSH_FP(1); break;
+            case 2: FSH_FP(2); break;
+            case 3: FSH_FP(3); break;
+            case 4: FSH_FP(4); break;
+            case 5: FSH_FP(5); break;
+            case 6: FSH_FP(6); break;
+            case 7: FSH_FP(7); break;
+            case 8: FSH_FP(8); break;
+            case 9: FSH_FP(9); break;
+            case 10: FSH_FP(10); break;
+            case 11: FSH_FP(11); break;
+            case 12: FSH_FP(12); break;
+            case 13: FSH_FP(13); break;
+            case 14: FSH_FP(14); break;
+            case 15: FSH_FP(15); break;
+            default: FSH_FP(16); break;
+        }
+#undef FSH_FP
+        FSH_TRY(hipGetLastError());
+        return OK;
     }
     const int budget = std::min(lds_limit(), 65536);
     const int max_rows = (budget - 4 * L) / 512;
@@ -678,6 +855,23 @@ int launch_final_tiles(const float2* data, int B, int Pf, int Qf, int T, const d
         return E_INVALID;
     }
     if (B < 1 || Pf < 1 || Qf < 1) { set_error("final_tiles: invalid shape"); return E_INVALID; }
+    const char* force = getenv("FSH_FINAL_KERNEL");
+    if (!(force && strcmp(force, "lds") == 0)) {
+        const int per = (T / std::min(T, 8)) * (T / std::min(T, 8));
+        const long blocks = ((long)B * per + 255) / 256;
+        const bool w = w0 != nullptr;
+#define FSH_FR(TT, WW) hipLaunchKernelGGL((k_final_tiles_reg<TT, WW>), dim3((unsigned)blocks), dim3(256), 0, s, data, B, Pf, Qf, ux, uy, \
+                                          tgeo, spx, spy, a0, a1, fc2, w0, gx, gyr, tile_xy, ox, oy, nx, ny, out)
+        switch (T) {
+            case 4: if (w) FSH_FR(4, true); else FSH_FR(4, false); break;
+            case 8: if (w) FSH_FR(8, true); else FSH_FR(8, false); break;
+            case 16: if (w) FSH_FR(16, true); else FSH_FR(16, false); break;
+            default: if (w) FSH_FR(32, true); else FSH_FR(32, false); break;
+        }
+#undef FSH_FR
+        FSH_TRY(hipGetLastError());
+        return OK;
+    }
     const int NT = 4096 / (T * T);
     const size_t lds = (size_t)2 * NT * T * T * 8;
     const long blocks = (B + NT - 1) / NT;

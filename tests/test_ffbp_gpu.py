@@ -50,8 +50,11 @@ def test_rot_fir_k_stage(D, L, Cn, Np, K, kernel, monkeypatch):
     assert e < (-120 if kernel == 'valu' else -100), e
 
 
-@pytest.mark.parametrize('D,pf', [(2, 0.5), (11, 0.7), (5, 0.3)])
-def test_fir_p_stage(D, pf):
+@pytest.mark.parametrize('kernel', ['stream', 'lds'])
+@pytest.mark.parametrize('D,pf', [(2, 0.5), (11, 0.7), (5, 0.3), (3, 0.9), (8, 0.6)])
+def test_fir_p_stage(D, pf, kernel, monkeypatch):
+    """pulse decimation (streaming sliding-window kernel and the LDS-staged one) against the float64 oracle"""
+    monkeypatch.setenv('FSH_FIRP_KERNEL', kernel)
     rng = np.random.default_rng(D)
     B, P, Ko = 3, 613, 70
     fr, _ = pl.decimator_fir(P, D, pf)
@@ -73,8 +76,12 @@ def _final_inputs(rng, B, Pf, Qf, T):
     return d, ux, uy, geo
 
 
+@pytest.mark.parametrize('weights', [False, True])
+@pytest.mark.parametrize('kernel', ['reg', 'lds'])
 @pytest.mark.parametrize('T', [4, 8, 16, 32])
-def test_final_tiles_stage(T):
+def test_final_tiles_stage(T, kernel, weights, monkeypatch):
+    """final stage (register-blocked and LDS-table kernels, with and without aperture weights) against float64"""
+    monkeypatch.setenv('FSH_FINAL_KERNEL', kernel)
     rng = np.random.default_rng(T)
     B, Pf, Qf = 5, 23, 41
     spx, spy = 0.5, 0.45
@@ -83,14 +90,20 @@ def test_final_tiles_stage(T):
     d, ux, uy, geo = _final_inputs(rng, B, Pf, Qf, T)
     dlx = (np.arange(T) - (T - 1) / 2.0) * spx
     dly = (np.arange(T) - (T - 1) / 2.0) * spy
-    ore, oim = oracle.final_tiles(d.real, d.imag, ux, uy, dlx, dly, a0, a1, geo[:, 0], geo[:, 1], geo[:, 2], fc2, T)
+    wts = None
+    if weights:
+        wts = (rng.uniform(0.5, 1.5, (B, Pf)), rng.uniform(-0.01, 0.01, (B, Pf)), rng.uniform(-0.01, 0.01, (B, Pf)))
+        wts = tuple(w.astype(np.float32) for w in wts)
+    ore, oim = oracle.final_tiles(d.real, d.imag, ux, uy, dlx, dly, a0, a1, geo[:, 0], geo[:, 1], geo[:, 2], fc2, T, wts)
     txy = np.stack([np.arange(B) * T, np.zeros(B)], 1).astype(np.int32)
     dd, dux, duy, dgeo, dtxy = (_hip.DeviceArray.from_host(a) for a in (d, ux, uy, geo, txy))
+    dw = [None] * 3 if wts is None else [_hip.DeviceArray.from_host(w) for w in wts]
     out = _hip.DeviceArray((B * T, T), np.complex64)
     _hip.check(_hip.lib().fsh_final_tiles(_hip.ptr(dd), B, Pf, Qf, T, _hip.ptr(dux), _hip.ptr(duy), _hip.ptr(dgeo), spx, spy,
-                                          a0, a1, fc2, None, None, None, _hip.ptr(dtxy), 0, 0, B * T, T, _hip.ptr(out), None))
+                                          a0, a1, fc2, *[_hip.ptr(w) for w in dw], _hip.ptr(dtxy), 0, 0, B * T, T,
+                                          _hip.ptr(out), None))
     e = db(out.to_host().reshape(B, T, T), ore + 1j * oim)
-    print(f'final_tiles T={T}: {e:.1f} dB')
+    print(f'final_tiles {kernel} T={T} weights={weights}: {e:.1f} dB')
     assert e < -110, e
 
 

@@ -24,7 +24,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = 's/image + Gpixel·pulses/s at 12288×8192, at ≤−60 dB error vs FP64 oracle'
-FP32_PEAK = 157.3e12          # MI355X published dense FP32 rate, FLOP/s
 HBM_PEAK = 8.0e12             # bytes/s
 
 
@@ -95,17 +94,32 @@ def cpu_baseline(col, grid, S_host, groups, T_cpu=16):
                 s_per_image=round(t_full, 3))
 
 
+def measured_traffic(label, former, args):
+    """HBM bytes per launch of `label` from the committed counter summary (FETCH_SIZE + WRITE_SIZE scaled by a copy
+    kernel's reading), when the summary covers this workload; else None."""
+    path = os.path.join(ROOT, 'profiles', f'ffbp_12288x8192_T{former.T}_hbm_traffic.md')
+    if args.small or former.levels != 4 or args.atten != 80.0 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith('traffic_bytes_per_launch '):
+            return json.loads(line.split(' ', 1)[1]).get(label)
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--T', type=int, default=8)
+    ap.add_argument('--atten', type=float, default=80.0, help='stopband attenuation of the decimation filters (dB)')
     ap.add_argument('--levels', type=int, default=4, help='levels of the plan (the former API default is 3, more when 3 cannot split)')
     ap.add_argument('--tiles', type=int, default=16, help='accuracy tiles (256 x 256) against float64 backprojection')
     ap.add_argument('--upsample', type=int, default=64, help='oversampling of the float64 checker profiles')
     ap.add_argument('--cpu-groups', type=int, default=2, help='groups of 8 first-level children timed on the CPU')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-profile', action='store_true', help='timed steps only (for counter runs under rocprofv3)')
+    ap.add_argument('--calibrate-copy', action='store_true', help='also run the plain copy kernel on 1 GiB three times')
     ap.add_argument('--small', action='store_true', help='the 512 x 512 scene instead (quick check)')
     args = ap.parse_args()
     if args.gpus != 1:
@@ -115,7 +129,7 @@ def main():
 
     from fastsar_amd import ImageFormer, _hip, sim
     from fastsar_amd.exact import simulate_device, bp_points_f64_device
-    from fastsar_amd.metrics import rel_db, stage_flops
+    from fastsar_amd.metrics import rel_db, stage_bytes, stage_flops
     from fastsar_amd.api import _window
 
     dev = _hip.device_info()
@@ -128,7 +142,8 @@ def main():
     log(f'phase history {col.Np} x {col.K} generated on the device in {time.perf_counter() - t0:.2f} s')
 
     t0 = time.perf_counter()
-    former = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T, levels=args.levels or None)
+    former = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T, levels=args.levels or None,
+                         atten=args.atten)
     plan = former._plan
     log(f'former built in {time.perf_counter() - t0:.2f} s: T={former.T}, levels={former.levels}, '
         f'splits={plan["split"]}, final {plan["final"]["P"]} x {plan["final"]["K"]}, predicted {former.predicted_error_db:.1f} dB')
@@ -148,6 +163,17 @@ def main():
     pairs = grid['nx'] * grid['ny'] * col.Np
     log(f'{dt * 1e3:.1f} ms per image, {pairs / dt / 1e9:.1f} Gpixel.pulses/s')
 
+    if args.calibrate_copy:
+        # the plain copy kernel on 1 GiB, three times: the byte counters' reference point
+        src, dst = _hip.DeviceArray((1 << 27,), np.complex64), _hip.DeviceArray((1 << 27,), np.complex64)
+        for _ in range(3):
+            _hip.check(_hip.lib().fsh_copy(_hip.ptr(dst), _hip.ptr(src), 1 << 30, None))
+        _hip.check(_hip.lib().fsh_synchronize())
+        log('copy kernel: 3 x 1 GiB')
+    if args.no_profile:
+        print(json.dumps(dict(metric=METRIC, value=round(pairs / dt / 1e9, 2), ms_per_step=round(dt * 1e3, 2))), flush=True)
+        return
+
     # per-launch device times (events on the former's stream) of one more call
     former._form.set_profile(True)
     former(dS, out=img)
@@ -158,31 +184,39 @@ def main():
         a = agg.setdefault((st, lvl), [0.0, 0])
         a[0] += ms
         a[1] += 1
-    flops = stage_flops(plan)
-    stages = {f'{k[0]}[{k[1]}]': dict(ms=round(v[0], 3), launches=v[1],
-                                      tflops=round(flops[k] / (v[0] * 1e-3) / 1e12, 2) if k in flops else None)
-              for k, v in sorted(agg.items(), key=lambda kv: -kv[1][0])}
+    flops, nbytes = stage_flops(plan), stage_bytes(plan)
+    stages = {}
+    for k, v in sorted(agg.items(), key=lambda kv: -kv[1][0]):
+        sec = v[0] * 1e-3
+        stages[f'{k[0]}[{k[1]}]'] = dict(ms=round(v[0], 3), launches=v[1],
+                                         tflops=round(flops[k] / sec / 1e12, 2) if k in flops else None,
+                                         gbs=round(nbytes[k] / sec / 1e9, 1) if k in nbytes else None)
     for k, v in stages.items():
         log(f'  {k}: {v}')
-    dom = max((k for k in agg if k in flops), key=lambda k: agg[k][0])
+    # the dominant kernel: the (stage, level) with the most device time; its roof is HBM (the intensities of every
+    # level and final kernel, flops over algorithmic bytes, sit below the ridge of the MFMA rate they run at)
+    dom = max((k for k in agg if k in nbytes), key=lambda k: agg[k][0])
+    label = f'{dom[0]}[{dom[1]}]'
     dom_ms = agg[dom][0] / agg[dom][1]
-    dom_flop = flops[dom] / agg[dom][1]
-    achieved = dom_flop / (dom_ms * 1e-3)
-    roofline = dict(bound='mfma', achieved=round(achieved / 1e12, 3), peak=FP32_PEAK / 1e12, unit='TFLOP/s',
-                    frac=round(achieved / FP32_PEAK, 4), traffic=None, kernel=f'{dom[0]} level {dom[1]}',
-                    flop_per_launch=dom_flop, ms_per_launch=round(dom_ms, 4))
+    dom_bytes = nbytes[dom] / agg[dom][1]
+    achieved = dom_bytes / (dom_ms * 1e-3)
+    roofline = dict(bound='hbm', achieved=round(achieved / 1e9, 1), peak=HBM_PEAK / 1e9, unit='GB/s',
+                    frac=round(achieved / HBM_PEAK, 4), traffic=measured_traffic(label, former, args),
+                    kernel=label, bytes_per_launch=round(dom_bytes), ms_per_launch=round(dom_ms, 4),
+                    flop_per_launch=round(flops.get(dom, 0.0) / agg[dom][1]),
+                    tflops=round(flops.get(dom, 0.0) / agg[dom][1] / (dom_ms * 1e-3) / 1e12, 2))
 
     # error against float64 exact backprojection on seeded tiles
     t0 = time.perf_counter()
     tiles, pts, side = accuracy_points(grid, args.tiles, 256, seed=7)
     nfft = 1 << int(np.ceil(np.log2(args.upsample * col.K)))
     wp, wk = _window(col.Np, col.K)
-    ref_vals = bp_points_f64_device(dS, col.ant, np.linalg.norm(col.ant, axis=1), col.fmin, col.df, pts, nfft, False, wp, wk,
+    ref_vals = bp_points_f64_device(dS, col.ant, np.linalg.norm(col.ant, axis=1), col.fmin, col.df, pts, nfft, True, wp, wk,
                                     chunk=max(1, (1 << 30) // (nfft * 16)))
     himg = img.to_host()
     got = np.concatenate([himg[i0:i0 + side, j0:j0 + side].ravel() for i0, j0 in tiles])
     err = rel_db(got, ref_vals)
-    log(f'error vs float64 exact backprojection ({len(tiles)} tiles of {side}^2, linear reads at {args.upsample}x): '
+    log(f'error vs float64 exact backprojection ({len(tiles)} tiles of {side}^2, cubic reads at {args.upsample}x): '
         f'{err:.2f} dB ({time.perf_counter() - t0:.1f} s)')
 
     cpu = None
@@ -200,7 +234,7 @@ def main():
                warmup=args.warmup, ms_per_step=round(dt * 1e3, 2), higher_is_better=True, scaling='weak',
                vs_baseline=None, dtype='f32', data='synthetic',
                config=dict(workload=(f'{"512x512 from 512x1024" if args.small else "12288x8192 image from 16384 pulses x 8192 samples"}'
-                                     f' complex64, factorized backprojection, T={former.T}, {former.levels} levels'),
+                                     f' complex64, factorized backprojection, T={former.T}, {former.levels} levels, {args.atten:g} dB filters'),
                            s_per_image=round(dt, 4), error_db=round(err, 2), error_tiles=f'{len(tiles)} x {side}^2',
                            predicted_error_db=round(former.predicted_error_db, 2), parallelism='single GPU'),
                roofline=roofline, cpu_baseline=cpu, stages=stages)

@@ -56,6 +56,7 @@ def lib():
         'fsh_memcpy_d2d': ([vp, vp, c_size_t], c_int),
         'fsh_memset': ([vp, c_int, c_size_t], c_int),
         'fsh_synchronize': ([], c_int),
+        'fsh_copy': ([vp, vp, c_size_t, vp], c_int),
         'fsh_rot_fir_k': ([vp, c_int, c_int, c_int, vp, vp, c_int, vp, c_int, c_int, c_int, c_int, c_double, vp, vp], c_int),
         'fsh_fir_p': ([vp, c_int, c_int, c_int, vp, c_int, c_int, c_int, c_int, vp, vp], c_int),
         'fsh_device_phases': ([vp, vp, vp, vp, c_int, c_int, c_int, c_double, c_double, vp, vp, vp], c_int),

@@ -153,11 +153,16 @@ class ImageFormer:
     T: 'auto' (the largest of 32 and 16 whose predicted error meets target_db) or 4, 8, 16, 32. levels: None
     (three, more where three cannot split an axis) or a count. aperture_weight(points, pulses) -> [n, m]: per-pixel
     pulse weights applied in the final stage. ref [P]: each pulse's reference range when it is not |ant|.
-    precision: 'float32'. range_gate: 'auto' or False (the hip first level holds any filter window)."""
+    precision: 'float32'. range_gate: 'auto' or False (the hip first level holds any filter window).
+    atten: stopband attenuation (dB) of the levels' Kaiser decimation filters; 70 (the reference's design, whose
+    plans this reproduces bit for bit) sets an image error floor near -60 dB, 80 lowers it by about 10 dB for
+    filters about 16 percent longer."""
 
     def __init__(self, ant, fmin, df, K, nx, ny, spx, spy, e1=(1.0, 0.0, 0.0), e2=(0.0, 1.0, 0.0), *, backend='auto',
                  precision='float32', window=True, T='auto', levels=None, pmax=0.4, target_db=-40.0, aperture_weight=None,
-                 ref=None, range_gate='auto', padded=None):
+                 ref=None, range_gate='auto', padded=None, atten=70.0):
+        if not (np.isfinite(atten) and 40.0 <= float(atten) <= 140.0):
+            raise ValueError(f'atten must be a stopband attenuation between 40 and 140 dB, got {atten!r}')
         self.ant = _check_positions(ant)
         self.P, self.K = self.ant.shape[0], int(K)
         if self.K < 2:
@@ -201,7 +206,7 @@ class ImageFormer:
                              f'pulse is from and an aperture within the {self.P} pulses, got {padded!r}')
         while True:
             try:
-                plan = _plan.make_plan(col, nx, ny, spx, spy, T=T, nlev=nlev, pmax=pmax, e1=e1, e2=e2,
+                plan = _plan.make_plan(col, nx, ny, spx, spy, T=T, nlev=nlev, pmax=pmax, atten=float(atten), e1=e1, e2=e2,
                                        pulse=padded and padded[2])
                 break
             except ValueError as e:

@@ -63,6 +63,12 @@ __global__ void k_fill_int(int* p, int n, int v)
     if (i < n) p[i] = v;
 }
 
+// plain device copy in 16-byte words: the reference point for HBM byte counters (bytes read = bytes written = n)
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
 int launch_fill_int(int* p, int n, int value, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, p, n, value);
@@ -313,6 +319,19 @@ int fsh_memcpy_d2h(void* dst, const void* src, size_t bytes) { FSH_TRY(hipMemcpy
 int fsh_memcpy_d2d(void* dst, const void* src, size_t bytes) { FSH_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice)); return OK; }
 int fsh_memset(void* dst, int value, size_t bytes) { FSH_TRY(hipMemset(dst, value, bytes)); return OK; }
 int fsh_synchronize(void) { FSH_TRY(hipDeviceSynchronize()); return OK; }
+
+int fsh_copy(void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (!dst || !src || (bytes & 15) || (((uintptr_t)dst | (uintptr_t)src) & 15)) {
+        set_error("copy: 16-byte aligned pointers and a multiple of 16 bytes");
+        return E_INVALID;
+    }
+    const size_t n16 = bytes / 16;
+    const unsigned blocks = (unsigned)std::min<size_t>((n16 + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const uint4*)src, (uint4*)dst, n16);
+    FSH_TRY(hipGetLastError());
+    return OK;
+}
 
 int fsh_rot_fir_k(const float* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps, int L,
                   int pl, int D, int Ko, double kc, float* y, void* stream)

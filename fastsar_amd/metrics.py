@@ -48,3 +48,20 @@ def min_bytes(plan):
     for N, lv in level_counts(plan):
         b += 2 * 8.0 * N * lv['C'] * lv['Po'] * lv['Ko']
     return b
+
+
+def stage_bytes(plan):
+    """Algorithmic HBM bytes of each (stage, level) over one image, complex64 samples and float32 rotations, each
+    array touched once:
+      rotation + range FIR  8 N P K (parent rows) + 8 N C P (c0, sl) + 8 N C P Ko (output)
+      pulse FIR             8 N C P Ko (input) + 8 N C Po Ko (output)
+      final stage           8 Pf Qf per tile (input) + 16 Pf per tile (ux, uy) + 8 nx ny (image)."""
+    out = {}
+    for l, (N, lv) in enumerate(level_counts(plan)):
+        out[('rot_fir_k', l)] = 8.0 * N * (lv['P'] * lv['K'] + lv['C'] * lv['P'] + lv['C'] * lv['P'] * lv['Ko'])
+        if lv['Dp'] > 1:
+            out[('fir_p', l)] = 8.0 * N * lv['C'] * (lv['P'] + lv['Po']) * lv['Ko']
+    f = plan['final']
+    ntiles = f['cen'].shape[0]
+    out[('final_tiles', len(plan['levels']))] = (8.0 * f['P'] * f['K'] + 16.0 * f['P']) * ntiles + 8.0 * plan['nx'] * plan['ny']
+    return out

@@ -46,6 +46,9 @@ int fsh_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int fsh_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int fsh_memset(void* dst, int value, size_t bytes);
 int fsh_synchronize(void);
+/* A plain device-to-device copy kernel (16-byte aligned, a multiple of 16 bytes): the reference point for the
+ * HBM byte counters of the other kernels. No reference counterpart. */
+int fsh_copy(void* dst, const void* src, size_t bytes, void* stream);
 
 /* ------------------------------------------------------------------------------------------ stages (device pointers)
  * stream: a hipStream_t, or NULL for the default stream. */

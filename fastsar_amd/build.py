@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, 'csrc')
 OUT = os.path.join(HERE, '_lib')
-SOURCES = ['fsh_api.hip', 'ffbp_kernels.hip', 'exact_kernels.hip', 'sim_kernels.hip']
+SOURCES = ['fsh_api.hip', 'ffbp_kernels.hip', 'fused_kernels.hip', 'exact_kernels.hip', 'sim_kernels.hip']
 ARCH = os.environ.get('FSH_ARCH', 'gfx950')
 
 
@@ -32,9 +32,9 @@ def flags():
 def build(verbose=False):
     """Compile every source and link libfsh.so; returns its path."""
     os.makedirs(OUT, exist_ok=True)
-    objs = []
+    objs, procs = [], []
     cc = hipcc()
-    for s in SOURCES:
+    for s in SOURCES:                      # the sources compile in parallel
         src = os.path.join(SRC, s)
         if not os.path.exists(src):
             continue
@@ -42,8 +42,11 @@ def build(verbose=False):
         cmd = [cc, *flags(), '-c', src, '-o', obj]
         if verbose:
             print(' '.join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        procs.append((subprocess.Popen(cmd), cmd))
         objs.append(obj)
+    failed = [cmd for pr, cmd in procs if pr.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     lib = os.path.join(OUT, 'libfsh.so')
     tmp = lib + '.tmp'
     cmd = [cc, f'--offload-arch={ARCH}', '-shared', '-fPIC', '-o', tmp, *objs, '-lhipfft']

@@ -99,10 +99,13 @@ struct Level {
     double* r0 = nullptr;
     double* d = nullptr;
     double* ref = nullptr;
+    float* hk_t = nullptr;         // polyphase tables of the fused form (fused levels only)
+    float* hp_t = nullptr;
+    bool fused = false;            // rotation and both filters in one pass (launch_level_fused)
     long long below = 1;           // tiles per parent... children per first-level child at this level's input
 };
 
-enum { ST_PREP, ST_ROT, ST_FIRP, ST_PHASE, ST_GEOM, ST_FINAL, ST_N };
+enum { ST_PREP, ST_ROT, ST_FIRP, ST_PHASE, ST_GEOM, ST_FINAL, ST_LEVEL, ST_N };
 
 struct Former {
     fsh_plan_desc p;
@@ -157,7 +160,7 @@ struct Former {
     {
         for (auto& l : lv) {
             dev_free(l.taps_k); dev_free(l.taps_p); dev_free(l.c0); dev_free(l.sl); dev_free(l.u); dev_free(l.r0);
-            dev_free(l.d); dev_free(l.ref);
+            dev_free(l.d); dev_free(l.ref); dev_free(l.hk_t); dev_free(l.hp_t);
         }
         lv.clear();
         dev_free(one); dev_free(wp); dev_free(wk); dev_free(fu); dev_free(fr0); dev_free(fcen); dev_free(tile_xy);
@@ -177,8 +180,8 @@ struct Former {
             const fsh_level_desc& h = lv[l].h;
             const long long ch = (l == 0) ? g : (long long)h.C;
             const long long outs = parents * ch;
-            maxbuf = std::max(maxbuf, (size_t)(outs * h.P * h.Ko));
-            if (h.Dp > 1) maxbuf = std::max(maxbuf, (size_t)(outs * h.Po * h.Ko));
+            if (!lv[l].fused) maxbuf = std::max(maxbuf, (size_t)(outs * h.P * h.Ko));
+            if (h.Dp > 1 || lv[l].fused) maxbuf = std::max(maxbuf, (size_t)(outs * h.Po * h.Ko));
             if (l >= 2) maxph = std::max(maxph, (size_t)(parents * h.C * h.P));
             parents = outs;
         }
@@ -250,6 +253,19 @@ struct Former {
                 mark_end(ST_PHASE);
                 c0 = c0b;
                 sl = slb;
+            }
+            if (L.fused) {
+                mark_begin();
+                if ((st = launch_level_fused(cur, (int)parents, h.P, h.K, c0, sl, ch, L.hk_t, h.Dk > 1 ? h.Lk : 1, h.Dk,
+                                             h.Dk > 1 ? h.plk : 0, h.Ko, L.hp_t, h.Dp > 1 ? h.Lp : 1, h.Dp, h.Dp > 1 ? h.plp : 0,
+                                             h.Po, h.kc, buf[yb], stream)))
+                    return st;
+                mark_end(ST_LEVEL);
+                cur = buf[yb];
+                curbuf = yb;
+                first_parent = (l == 0) ? g0 : first_parent * h.C;
+                parents *= ch;
+                continue;
             }
             mark_begin();
             if ((st = launch_rot_fir_k(cur, (int)parents, h.P, h.K, c0, sl, ch, h.Dk > 1 ? L.taps_k : one, h.Dk > 1 ? h.Lk : 1,
@@ -344,6 +360,34 @@ int fsh_fir_p(const float* y, int B, int P, int Ko, const float* taps, int L, in
     return launch_fir_p((const float2*)y, B, P, Ko, taps, L, pl, D, Po, (float2*)z, (hipStream_t)stream);
 }
 
+int fsh_level_fused(const float* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps_k,
+                    int Lk, int plk, int Dk, int Ko, const float* taps_p, int Lp, int plp, int Dp, int Po, double kc, float* z,
+                    void* stream)
+{
+    if (!x || !c0 || !sl || !z || (Dk > 1 && !taps_k) || (Dp > 1 && !taps_p)) { set_error("level_fused: null argument"); return E_INVALID; }
+    if (Dk == 1) { Lk = 1; plk = 0; taps_k = nullptr; }
+    if (Dp == 1) { Lp = 1; plp = 0; taps_p = nullptr; }
+    int CB, R;
+    size_t lds;
+    if (Np < 1 || Lk < 1 || Lp < 1 || Dk < 1 || Dp < 1 || !level_fused_shape(Cn, P, K, Lk, Dk, Ko, Lp, Dp, CB, R, lds)) {
+        set_error("level_fused: the level does not fit the fused form (Ko <= 256, D <= 4 and ceil(L / D) <= 16 per axis)");
+        return E_INVALID;
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    float* t = nullptr;
+    const size_t nk = (size_t)Dk * fused_nw(Lk, Dk), np = (size_t)Dp * fused_nw(Lp, Dp);
+    int st = dev_alloc((void**)&t, (nk + np) * sizeof(float));
+    if (st) return st;
+    if (!(st = launch_polyphase(taps_k, Lk, Dk, t, s)) && !(st = launch_polyphase(taps_p, Lp, Dp, t + nk, s)) &&
+        !(st = launch_level_fused((const float2*)x, Np, P, K, c0, sl, Cn, t, Lk, Dk, plk, Ko, t + nk, Lp, Dp, plp, Po, kc,
+                                  (float2*)z, s))) {
+        const hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_status(e, "level_fused");
+    }
+    dev_free(t);
+    return st;
+}
+
 int fsh_device_phases(const double* refs, const double* u, const double* r0, const double* d, int Np, int Cn, int P,
                       double k0c, double k1, float* c0, float* sl, void* stream)
 {
@@ -401,8 +445,11 @@ int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, voi
             set_error("ffbp_create: invalid level " + std::to_string(l));
             return fail(E_INVALID);
         }
-        Level L;
+        f->lv.emplace_back();          // owned by the former from here, so a failure below frees what was uploaded
+        Level& L = f->lv.back();
         L.h = h;
+        L.h.taps_k = L.h.taps_p = L.h.c0 = L.h.sl = nullptr;
+        L.h.u = L.h.r0 = L.h.d = L.h.ref = nullptr;
         if ((st = upload(&L.taps_k, h.taps_k, h.Dk > 1 ? (size_t)h.Lk : 0))) return fail(st);
         if ((st = upload(&L.taps_p, h.taps_p, h.Dp > 1 ? (size_t)h.Lp : 0))) return fail(st);
         if (h.c0) {
@@ -413,11 +460,20 @@ int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, voi
                 (st = upload(&L.d, h.d, (size_t)h.C * 3)) || (st = upload(&L.ref, h.ref, (size_t)h.nparents * 3)))
                 return fail(st);
         }
+        {
+            // later levels take the fused form where it applies (level 0's children come in groups)
+            const int Lk = h.Dk > 1 ? h.Lk : 1, Lp = h.Dp > 1 ? h.Lp : 1;
+            if (l > 0 && level_fused_preferred(h.C, h.P, h.K, Lk, h.Dk, h.Ko, Lp, h.Dp)) {
+                if ((st = dev_alloc((void**)&L.hk_t, (size_t)h.Dk * fused_nw(Lk, h.Dk) * sizeof(float))) ||
+                    (st = dev_alloc((void**)&L.hp_t, (size_t)h.Dp * fused_nw(Lp, h.Dp) * sizeof(float))) ||
+                    (st = launch_polyphase(h.Dk > 1 ? L.taps_k : nullptr, Lk, h.Dk, L.hk_t, f->stream)) ||
+                    (st = launch_polyphase(h.Dp > 1 ? L.taps_p : nullptr, Lp, h.Dp, L.hp_t, f->stream)))
+                    return fail(st);
+                L.fused = true;
+            }
+        }
         L.below = children;
         children *= h.C;
-        L.h.taps_k = L.h.taps_p = L.h.c0 = L.h.sl = nullptr;
-        L.h.u = L.h.r0 = L.h.d = L.h.ref = nullptr;
-        f->lv.push_back(L);
     }
     if (children != p.ntiles) {
         set_error("ffbp_create: the levels' children do not multiply to the tile count");

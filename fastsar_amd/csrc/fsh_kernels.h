@@ -11,6 +11,15 @@ int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, con
                      int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s);
 int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float2* z,
                  hipStream_t s);
+// the rotation and both filters of a level in one pass, for short children (fused_kernels.hip); hk_t and hp_t are the
+// polyphase tables that launch_polyphase builds from the taps (h == nullptr: the unit tap of an undecimated axis)
+int fused_nw(int L, int D);
+bool level_fused_shape(int C, int P, int K, int Lk, int Dk, int Ko, int Lp, int Dp, int& CB, int& R, size_t& lds);
+bool level_fused_preferred(int C, int P, int K, int Lk, int Dk, int Ko, int Lp, int Dp);
+int launch_polyphase(const float* h, int L, int D, float* t, hipStream_t s);
+int launch_level_fused(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int C, const float* hk_t,
+                       int Lk, int Dk, int plk, int Ko, const float* hp_t, int Lp, int Dp, int plp, int Po, double kc,
+                       float2* z, hipStream_t s);
 int launch_device_phases(const double* refs, const double* u, const double* r0, const double* d, int Np, int Cn, int P,
                          double k0c, double k1, float* c0, float* sl, hipStream_t s);
 int launch_final_geom(const double* fu, const double* fr0, const double* cen, int B, int Pf, const double* e1,

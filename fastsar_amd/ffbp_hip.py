@@ -13,7 +13,7 @@ import numpy as np
 from . import _hip
 from .plan import C, HOST_LEVELS, level_phase_constants, tile_digits
 
-STAGES = ('prepare', 'rot_fir_k', 'fir_p', 'device_phases', 'final_geom', 'final_tiles')
+STAGES = ('prepare', 'rot_fir_k', 'fir_p', 'device_phases', 'final_geom', 'final_tiles', 'level_fused')
 
 
 def _f32(a):

@@ -27,7 +27,8 @@ def stage_flops(plan):
       rotation + range FIR  6 N C P K + 4 N C P Ko Lk      (one complex multiply per sample per child; a real tap
                                                            times a complex sample per output and tap)
       pulse FIR             4 N C Po Ko Lp
-      final stage           8 T^2 Pf Qf per tile (a real 2T x PfQf x 2T product)."""
+      final stage           8 T^2 Pf Qf per tile (a real 2T x PfQf x 2T product)
+    and ('level_fused', l), the level's rotation and both filters in one pass, their sum."""
     out = {}
     for l, (N, lv) in enumerate(level_counts(plan)):
         rot = 6.0 * N * lv['C'] * lv['P'] * lv['K']
@@ -35,6 +36,7 @@ def stage_flops(plan):
         out[('rot_fir_k', l)] = rot + fk
         if lv['Dp'] > 1:
             out[('fir_p', l)] = 4.0 * N * lv['C'] * lv['Po'] * lv['Ko'] * lv['fir_p']['L']
+        out[('level_fused', l)] = out[('rot_fir_k', l)] + out.get(('fir_p', l), 0.0)
     f = plan['final']
     ntiles = f['cen'].shape[0]
     out[('final_tiles', len(plan['levels']))] = 8.0 * plan['T'] ** 2 * f['P'] * f['K'] * ntiles
@@ -55,12 +57,14 @@ def stage_bytes(plan):
     array touched once:
       rotation + range FIR  8 N P K (parent rows) + 8 N C P (c0, sl) + 8 N C P Ko (output)
       pulse FIR             8 N C P Ko (input) + 8 N C Po Ko (output)
-      final stage           8 Pf Qf per tile (input) + 16 Pf per tile (ux, uy) + 8 nx ny (image)."""
+      final stage           8 Pf Qf per tile (input) + 16 Pf per tile (ux, uy) + 8 nx ny (image)
+      fused level           8 N P K + 8 N C P + 8 N C Po Ko (parent rows, rotations, output)."""
     out = {}
     for l, (N, lv) in enumerate(level_counts(plan)):
         out[('rot_fir_k', l)] = 8.0 * N * (lv['P'] * lv['K'] + lv['C'] * lv['P'] + lv['C'] * lv['P'] * lv['Ko'])
         if lv['Dp'] > 1:
             out[('fir_p', l)] = 8.0 * N * lv['C'] * (lv['P'] + lv['Po']) * lv['Ko']
+        out[('level_fused', l)] = 8.0 * N * (lv['P'] * lv['K'] + lv['C'] * lv['P'] + lv['C'] * lv['Po'] * lv['Ko'])
     f = plan['final']
     ntiles = f['cen'].shape[0]
     out[('final_tiles', len(plan['levels']))] = (8.0 * f['P'] * f['K'] + 16.0 * f['P']) * ntiles + 8.0 * plan['nx'] * plan['ny']

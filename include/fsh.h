@@ -65,6 +65,15 @@ int fsh_rot_fir_k(const float* x, int Np, int P, int K, const float* c0, const f
 int fsh_fir_p(const float* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float* z,
               void* stream);
 
+/* One level in one pass: fsh_rot_fir_k followed by fsh_fir_p without the intermediate y, for short children
+ * (Ko <= 256, D <= 4 and ceil(L / D) <= 16 on both axes; FSH_EINVAL otherwise). x [Np, P, K], c0 and sl [Np, Cn, P], taps_k [Lk]
+ * (NULL when Dk == 1), taps_p [Lp] (NULL when Dp == 1), z [Np * Cn, Po, Ko]; synchronizes the stream. Replaces the pair
+ * rot_fir_k_cplx + fir_p of a later level (fastsar/ffbp_cpu.cpp:244-293, called per child in
+ * fastsar/ffbp_cpu.py:65-91). */
+int fsh_level_fused(const float* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps_k,
+                    int Lk, int plk, int Dk, int Ko, const float* taps_p, int Lp, int plp, int Dp, int Po, double kc, float* z,
+                    void* stream);
+
 /* Child rotations of a level computed on the device in float64: refs [Np, 3] parent centres, u [P, 3], r0 [P] the
  * level's track, d [Cn, 3] child offsets -> c0, sl [Np, Cn, P] float32. Replaces the device-phase kernel and the
  * host form (fastsar/ffbp_cuda.py:598-632, fastsar/ffbp_cpu.py:142-155). */
@@ -127,7 +136,8 @@ int fsh_ffbp_form(void* handle, const void* S, int S_on_device, void* image, int
 int fsh_ffbp_set_group(void* handle, int ng);
 /* The group used by the last call and the device bytes the former holds. */
 int fsh_ffbp_info(void* handle, int* ng_used, size_t* device_bytes);
-/* Per-stage device times (ms) of the last call: prepare, rot_fir_k, fir_p, device_phases, final_geom, final_tiles. */
+/* Per-stage device times (ms) of the last call: prepare, rot_fir_k, fir_p, device_phases, final_geom, final_tiles,
+ * level_fused. */
 int fsh_ffbp_stage_ms(void* handle, float* ms, int n);
 /* Profiling (no reference counterpart; the analog of ffbp_cuda.PROFILE, fastsar/ffbp_cuda.py:635-652): when on,
  * every launch is bracketed by events on the former's stream and waited for; fsh_ffbp_profile then gives, per

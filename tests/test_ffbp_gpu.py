@@ -68,6 +68,50 @@ def test_fir_p_stage(D, pf, kernel, monkeypatch):
     assert e < -125, e
 
 
+FUSED_CASES = [(2, 0.6, 3, 0.6, 6, 2, 198, 50), (4, 0.6, 4, 0.6, 5, 1, 300, 315), (1, 0, 2, 0.5, 3, 2, 100, 40),
+               (3, 0.5, 1, 0, 4, 1, 77, 90), (4, 0.6, 3, 0.7, 17, 3, 150, 200), (2, 0.4, 4, 0.7, 8, 1, 613, 30),
+               (2, 0.5, 2, 0.5, 4, 9, 143, 183)]
+
+
+@pytest.mark.parametrize('Dk,pk,Dp,pp,Cn,Np,P,K', FUSED_CASES)
+def test_level_fused_stage(Dk, pk, Dp, pp, Cn, Np, P, K):
+    """a level's rotation and both filters in one pass against the float64 oracle's two stages"""
+    rng = np.random.default_rng(Dk * 1000 + Dp * 100 + Cn)
+    x = (rng.standard_normal((Np, P, K)) + 1j * rng.standard_normal((Np, P, K))).astype(np.complex64)
+    c0 = rng.uniform(-0.5, 0.5, (Np, Cn, P)).astype(np.float32)
+    sl = rng.uniform(-0.8, 0.8, (Np, Cn, P)).astype(np.float32)
+    frk = pl.decimator_fir(K, Dk, pk)[0] if Dk > 1 else None
+    frp = pl.decimator_fir(P, Dp, pp)[0] if Dp > 1 else None
+    Ko = frk['n_out'] if frk else K
+    Po = frp['n_out'] if frp else P
+    kc = (K - 1) / 2.0
+    yre, yim = oracle.rot_fir_k(x.real, x.imag, c0, sl, frk, Dk, Ko, kc)
+    if frp:
+        zre, zim = oracle.fir_p(yre, yim, frp, Dp, Po)
+    else:
+        zre, zim = yre, yim
+    tk = frk['kern'].astype(np.float32) if frk else np.ones(1, np.float32)
+    tp = frp['kern'].astype(np.float32) if frp else np.ones(1, np.float32)
+    dx, dc, ds, dtk, dtp = (_hip.DeviceArray.from_host(a) for a in (x, c0, sl, tk, tp))
+    dz = _hip.DeviceArray((Np * Cn, Po, Ko), np.complex64)
+    print(f'level_fused Dk={Dk} Dp={Dp} C={Cn} Np={Np} P={P} K={K} -> [{Po} x {Ko}]', flush=True)
+    _hip.check(_hip.lib().fsh_level_fused(_hip.ptr(dx), Np, P, K, _hip.ptr(dc), _hip.ptr(ds), Cn, _hip.ptr(dtk), len(tk),
+                                          frk['pl'] if frk else 0, Dk, Ko, _hip.ptr(dtp), len(tp), frp['pl'] if frp else 0,
+                                          Dp, Po, kc, _hip.ptr(dz), None), 'level_fused')
+    e = db(dz.to_host(), zre + 1j * zim)
+    print(f'  {e:.1f} dB')
+    assert e < -115, e
+
+
+def test_level_fused_rejects():
+    """shapes outside the fused form are refused before any launch"""
+    lib = _hip.lib()
+    dz = _hip.DeviceArray((1,), np.complex64)
+    p = _hip.ptr(dz)
+    assert lib.fsh_level_fused(p, 1, 10, 20, p, p, 1, p, 25, 12, 5, 8, p, 3, 1, 2, 6, 9.5, p, None) == 2   # Dk = 5
+    assert lib.fsh_level_fused(p, 1, 10, 600, p, p, 1, None, 1, 0, 1, 600, None, 1, 0, 1, 10, 9.5, p, None) == 2   # Ko > 256
+
+
 def _final_inputs(rng, B, Pf, Qf, T):
     d = (rng.standard_normal((B, Pf, Qf)) + 1j * rng.standard_normal((B, Pf, Qf))).astype(np.complex64)
     ux = rng.uniform(-0.1, 0.1, (B, Pf)) + 0.05
@@ -135,11 +179,13 @@ def scene():
     return col, sim.simulate_brute(col, pos, amp).astype(np.complex64)
 
 
-@pytest.mark.parametrize('kernel', ['valu', 'mfma'])
+@pytest.mark.parametrize('kernel', ['valu', 'mfma', 'fused'])
 @pytest.mark.parametrize('nlev,T', [(2, 16), (3, 16), (3, 8), (2, 32), (4, 4)])
 def test_image_vs_oracle(scene, nlev, T, kernel, monkeypatch):
-    """whole images (float32 VALU levels, or bfloat16 x 3 MFMA levels) against the float64 oracle's image"""
-    monkeypatch.setenv('FSH_ROT_KERNEL', kernel)
+    """whole images (float32 VALU levels, bfloat16 x 3 MFMA levels, or MFMA level 0 and fused later levels) against
+    the float64 oracle's image"""
+    monkeypatch.setenv('FSH_ROT_KERNEL', 'valu' if kernel == 'valu' else 'mfma')
+    monkeypatch.setenv('FSH_LEVEL_KERNEL', 'fused' if kernel == 'fused' else 'split')
     col, S = scene
     e1, e2 = (0.0, 1.0, 0.0), (1.0, 0.0, 0.0)
     p = pl.make_plan(col, 128, 128, 0.5, 0.5, T=T, nlev=nlev, e1=e1, e2=e2)

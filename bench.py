@@ -117,6 +117,7 @@ def main():
     ap.add_argument('--tiles', type=int, default=16, help='accuracy tiles (256 x 256) against float64 backprojection')
     ap.add_argument('--upsample', type=int, default=64, help='oversampling of the float64 checker profiles')
     ap.add_argument('--cpu-groups', type=int, default=2, help='groups of 8 first-level children timed on the CPU')
+    ap.add_argument('--host-steps', type=int, default=2, help='also time calls from a host history to a host image')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-profile', action='store_true', help='timed steps only (for counter runs under rocprofv3)')
     ap.add_argument('--calibrate-copy', action='store_true', help='also run the plain copy kernel on 1 GiB three times')
@@ -162,6 +163,18 @@ def main():
     dt = (time.perf_counter() - t0) / args.steps
     pairs = grid['nx'] * grid['ny'] * col.Np
     log(f'{dt * 1e3:.1f} ms per image, {pairs / dt / 1e9:.1f} Gpixel.pulses/s')
+
+    # host to host (pageable history in, image out): the PCIe-inclusive figure, reported beside the device rate
+    h2h = None
+    if args.host_steps > 0 and not args.no_profile:
+        S_host = dS.to_host()
+        former(S_host)
+        t0 = time.perf_counter()
+        for i in range(args.host_steps):
+            former(S_host)
+        h2h = (time.perf_counter() - t0) / args.host_steps
+        del S_host
+        log(f'host to host: {h2h * 1e3:.1f} ms per image ({pairs / h2h / 1e9:.1f} Gpixel.pulses/s)')
 
     if args.calibrate_copy:
         # the plain copy kernel on 1 GiB, three times: the byte counters' reference point
@@ -236,7 +249,8 @@ def main():
                config=dict(workload=(f'{"512x512 from 512x1024" if args.small else "12288x8192 image from 16384 pulses x 8192 samples"}'
                                      f' complex64, factorized backprojection, T={former.T}, {former.levels} levels, {args.atten:g} dB filters'),
                            s_per_image=round(dt, 4), error_db=round(err, 2), error_tiles=f'{len(tiles)} x {side}^2',
-                           predicted_error_db=round(former.predicted_error_db, 2), parallelism='single GPU'),
+                           predicted_error_db=round(former.predicted_error_db, 2), parallelism='single GPU',
+                           host_to_host_ms=None if h2h is None else round(h2h * 1e3, 1)),
                roofline=roofline, cpu_baseline=cpu, stages=stages)
     print(json.dumps(out), flush=True)
 

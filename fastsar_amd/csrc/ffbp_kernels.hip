@@ -148,7 +148,7 @@ __device__ __forceinline__ void split_bf16(float v, __bf16& hi, __bf16& lo)
 
 __device__ __forceinline__ int skew(int i) { return i + (i >> 5); }
 
-template <int KS>
+template <int KS, bool SPLIT>
 __global__ __launch_bounds__(256) void k_rot_fir_k_mfma(const float2* __restrict__ x, int P, int K, const float* __restrict__ c0,
                                                          const float* __restrict__ sl, int Cn, const float* __restrict__ taps,
                                                          int L, int pl, int D, int Ko, double kc, int MC, int WIN,
@@ -222,7 +222,8 @@ __global__ __launch_bounds__(256) void k_rot_fir_k_mfma(const float2* __restrict
         cis_cyc(b * (double)D, sDr, sDi);
         cis_cyc(b * (double)(8 * D), s8r, s8i);
         for (int mt = m0; mt < m1; ++mt) {
-            floatx16_t acc = {};
+            // one accumulation chain, or three independent ones (hi hi, hi lo, lo hi) summed at the end
+            floatx16_t acc = {}, acc1 = {}, acc2 = {};
             const int base = D * ((mt - m0) * 32 + col) + 4 * q;
             #pragma unroll
             for (int s = 0; s < KS; ++s) {
@@ -232,8 +233,17 @@ __global__ __launch_bounds__(256) void k_rot_fir_k_mfma(const float2* __restrict
                 wl.x = XL[skew(i0)]; wl.y = XL[skew(i0 + 1)]; wl.z = XL[skew(i0 + 2)]; wl.w = XL[skew(i0 + 3)];
                 const bf16x8_t Ah = __builtin_bit_cast(bf16x8_t, wh), Al = __builtin_bit_cast(bf16x8_t, wl);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gh[s], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gl[s], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Gh[s], acc, 0, 0, 0);
+                if (SPLIT) {
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gl[s], acc1, 0, 0, 0);
+                    acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Gh[s], acc2, 0, 0, 0);
+                } else {
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gl[s], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Gh[s], acc, 0, 0, 0);
+                }
+            }
+            if (SPLIT) {
+                #pragma unroll
+                for (int i = 0; i < 16; ++i) acc[i] += acc1[i] + acc2[i];
             }
             // rho at rows 4q, 4q + 1, ... of this tile: seeded at row 4q, stepped by D samples, 8 rows apart by 8 D
             float rr, ri;
@@ -244,11 +254,10 @@ __global__ __launch_bounds__(256) void k_rot_fir_k_mfma(const float2* __restrict
                 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int i = 4 * g + e;
+                    // lane pair (t = 0, 1) holds (Re, Im) of one output: Re' = Re er - Im ei, Im' = Im er + Re ei
                     const float mine = acc[i];
                     const float other = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, mine), 0xB1, 0xF, 0xF, false));
-                    const float yr = t ? other : mine, yi = t ? mine : other;
-                    const float outr = fmaf(yr, er, -(yi * ei)), outi = fmaf(yr, ei, yi * er);
-                    stg[(col >> 1) * 66 + 2 * (8 * g + 4 * q + e) + t] = t ? outi : outr;
+                    stg[(col >> 1) * 66 + 2 * (8 * g + 4 * q + e) + t] = fmaf(mine, er, other * (t ? ei : -ei));
                     const float nr = fmaf(er, sDr, -(ei * sDi));
                     ei = fmaf(er, sDi, ei * sDr);
                     er = nr;
@@ -257,7 +266,7 @@ __global__ __launch_bounds__(256) void k_rot_fir_k_mfma(const float2* __restrict
                 ri = fmaf(rr, s8i, ri * s8r);
                 rr = nr;
             }
-            __syncthreads();
+            wave_lds_sync();
             const int j = mt * 32 + col;
             #pragma unroll
             for (int it = 0; it < 8; ++it) {
@@ -267,7 +276,7 @@ __global__ __launch_bounds__(256) void k_rot_fir_k_mfma(const float2* __restrict
                     y[(((size_t)n * Cn + cc) * P + p) * Ko + j] = v;
                 }
             }
-            __syncthreads();
+            wave_lds_sync();                 // the next tile's writes to stg come after these reads
         }
     }
 }
@@ -286,6 +295,8 @@ int launch_rot_fir_k_mfma(const float2* x, int Np, int P, int K, const float* c0
     const size_t lds = (size_t)4 * (4 * (2 * WS + MF_STAGE)) + 4 * 8 * KS;
     if (lds > 160 * 1024) return E_INVALID;
     const int gx = (P + 3) / 4, gy = (MT + MC - 1) / MC;
+    const char* acc_env = getenv("FSH_MFMA_CHAINS");      // 3: three independent accumulation chains
+    const bool split = acc_env && atoi(acc_env) == 3;
     for (int n0 = 0; n0 < Np; n0 += 65535) {
         const int nn = std::min(65535, Np - n0);
         dim3 grid(gx, gy, nn);
@@ -293,7 +304,11 @@ int launch_rot_fir_k_mfma(const float2* x, int Np, int P, int K, const float* c0
         const float* c0s = c0 + (size_t)n0 * Cn * P;
         const float* sls = sl + (size_t)n0 * Cn * P;
         float2* ys = y + (size_t)n0 * Cn * P * Ko;
-#define FSH_MF(KK) hipLaunchKernelGGL(k_rot_fir_k_mfma<KK>, grid, dim3(256), lds, s, xs, P, K, c0s, sls, Cn, taps, L, pl, D, Ko, kc, MC, WIN, ys)
+#define FSH_MF(KK)                                                                                                       \
+    do {                                                                                                                 \
+        if (split) hipLaunchKernelGGL((k_rot_fir_k_mfma<KK, true>), grid, dim3(256), lds, s, xs, P, K, c0s, sls, Cn, taps, L, pl, D, Ko, kc, MC, WIN, ys); \
+        else hipLaunchKernelGGL((k_rot_fir_k_mfma<KK, false>), grid, dim3(256), lds, s, xs, P, K, c0s, sls, Cn, taps, L, pl, D, Ko, kc, MC, WIN, ys);     \
+    } while (0)
         switch (KS) {
             case 1: FSH_MF(1); break;
             case 2: FSH_MF(2); break;
@@ -358,16 +373,19 @@ __global__ __launch_bounds__(256) void k_fir_p(const float2* __restrict__ y, int
 // Streaming form: one thread per column (b, j) and run of IC outputs; the input rows are read once, in order, and
 // each is added into the NW = ceil(L / D) outputs whose taps cover it, held as a sliding window of accumulators. Row
 // p = D m - pl + s (period m, phase s < D) feeds output i with tap D (m - i) + s; at the end of period m output
-// m - NW + 1 is complete, is stored, and the window shifts. Taps are zero padded to D NW in LDS. Consecutive
-// threads take consecutive columns, so every row load is coalesced. grid (ceil(B Ko / 256), ceil(Po / IC)).
-template <int NW>
+// m - NW + 1 is complete, is stored, and the window shifts. The taps are the same for every lane (scalar loads, zero
+// beyond L). Consecutive threads take consecutive columns, so every row load is coalesced.
+// grid (ceil(B Ko / 256), ceil(Po / IC)).
+template <int NW, bool BATCH>
 __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__ y, int B, int P, int Ko,
                                                        const float* __restrict__ taps, int L, int pl, int D, int Po, int IC,
                                                        float2* __restrict__ z)
 {
-    extern __shared__ float tpz[];                   // [D * NW], zero beyond L
-    for (int r = threadIdx.x; r < D * NW; r += 256) tpz[r] = r < L ? taps[r] : 0.f;
-    __syncthreads();
+    extern __shared__ float tpz[];                   // [D * NW], zero beyond L (LDS form)
+    if (!BATCH) {
+        for (int r = threadIdx.x; r < D * NW; r += 256) tpz[r] = r < L ? taps[r] : 0.f;
+        __syncthreads();
+    }
     const long col = (long)blockIdx.x * 256 + threadIdx.x;
     if (col >= (long)B * Ko) return;
     const int b = (int)(col / Ko), j = (int)(col - (long)b * Ko);
@@ -381,14 +399,40 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
     // from i0 to i1 + NW - 2 and output m - NW + 1 is stored at the end of period m when it lies in [i0, i1)
     for (int m = i0; m < i1 + NW - 1; ++m) {
         const int pbase = D * m - pl;
-        for (int ph = 0; ph < D; ++ph) {
-            const int p = pbase + ph;
-            const float2 v = (p >= 0 && p < P) ? yb[(size_t)p * Ko] : make_float2(0.f, 0.f);
-            #pragma unroll
-            for (int s = 0; s < NW; ++s) {
-                const float h = tpz[D * (NW - 1 - s) + ph];
-                acc[s].x = fmaf(h, v.x, acc[s].x);
-                acc[s].y = fmaf(h, v.y, acc[s].y);
+        if (!BATCH) {
+            for (int ph = 0; ph < D; ++ph) {
+                const int p = pbase + ph;
+                const float2 v = (p >= 0 && p < P) ? yb[(size_t)p * Ko] : make_float2(0.f, 0.f);
+                #pragma unroll
+                for (int s = 0; s < NW; ++s) {
+                    const float h = tpz[D * (NW - 1 - s) + ph];
+                    acc[s].x = fmaf(h, v.x, acc[s].x);
+                    acc[s].y = fmaf(h, v.y, acc[s].y);
+                }
+            }
+        } else {
+            // rows in batches of four, all four loads in flight before the sums (addresses clamped into the
+            // column, rows outside [0, P) or past the period's end contribute zero); taps by scalar loads
+            for (int ph0 = 0; ph0 < D; ph0 += 4) {
+                float2 v[4];
+                #pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int p = pbase + ph0 + u;
+                    v[u] = yb[(size_t)min(max(p, 0), P - 1) * Ko];
+                }
+                #pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int ph = ph0 + u, p = pbase + ph;
+                    if (ph < D && p >= 0 && p < P) {
+                        #pragma unroll
+                        for (int s = 0; s < NW; ++s) {
+                            const int r = D * (NW - 1 - s) + ph;
+                            const float h = r < L ? taps[r] : 0.f;
+                            acc[s].x = fmaf(h, v[u].x, acc[s].x);
+                            acc[s].y = fmaf(h, v[u].y, acc[s].y);
+                        }
+                    }
+                }
             }
         }
         const int done = m - NW + 1;
@@ -775,8 +819,13 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
     if (NW <= 16 && !(force && strcmp(force, "lds") == 0)) {
         const int IC = std::min(Po, 128);
         dim3 grid((unsigned)(((long)B * Ko + 255) / 256), (Po + IC - 1) / IC);
-        const size_t lds = (size_t)4 * D * NW;
-#define FSH_FP(NN) hipLaunchKernelGGL(k_fir_p_stream<NN>, grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z)
+        const bool batch = force && strcmp(force, "stream4") == 0;
+        const size_t lds = batch ? 0 : (size_t)4 * D * NW;
+#define FSH_FP(NN)                                                                                                       \
+    do {                                                                                                                 \
+        if (batch) hipLaunchKernelGGL((k_fir_p_stream<NN, true>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z); \
+        else hipLaunchKernelGGL((k_fir_p_stream<NN, false>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z);     \
+    } while (0)
         switch (NW) {
             case 1: FSH_FP(1); break;
             case 2: FSH_FP(2); break;

@@ -34,6 +34,17 @@ __device__ __forceinline__ void cis_cyc(double c, float& cs, float& sn)
     sincospif(2.0f * w, &sn, &cs);
 }
 
+// Orders one wave's LDS accesses across its lanes: writes before the barrier are seen by reads after it, and reads
+// before it complete before writes after it. A wave's LDS instructions execute in issue order, so this only keeps
+// the compiler from moving accesses across it (wavefront-scope fences); it does not wait for other waves.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    asm volatile("" ::: "memory");
+}
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
     return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));

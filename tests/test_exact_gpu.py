@@ -94,9 +94,9 @@ def test_bp_points_f64_device(interp):
     want = oracle.bp_points(S, col.ant, col.fmin, col.df, pts, upsample=16, interp=interp)
     e = oracle.plain_db(got, want)
     print(f'bp_points_f64 {interp}: {e:.1f} dB')
-    assert e < -200, e
+    assert e < -180, e
     b = backproject(S, col.ant, col.fmin, col.df, pts, upsample=16, interp=interp, chunk=64, window=False)
-    assert oracle.plain_db(b, want) < -200
+    assert oracle.plain_db(b, want) < -180
     wp, wk = oracle.window(*S.shape)
     got = bp_points_f64_device(dS, col.ant, ref, col.fmin, col.df, pts, nfft, interp == 'cubic', wp, wk, chunk=100)
     want = oracle.bp_points(S * wp[:, None] * wk[None, :], col.ant, col.fmin, col.df, pts, upsample=16, interp=interp)

@@ -34,9 +34,12 @@ int dev_alloc(void** p, size_t bytes)
 {
     *p = nullptr;
     if (bytes == 0) bytes = 256;
+    // test hooks: FSH_FAIL_ALLOC=n fails the n-th allocation of the process, FSH_FAIL_ALLOC_ABOVE=b every allocation
+    // of more than b bytes (a smaller device)
     const char* fail = getenv("FSH_FAIL_ALLOC");
+    const char* above = getenv("FSH_FAIL_ALLOC_ABOVE");
     const long n = ++g_alloc_count;
-    if (fail && atol(fail) > 0 && n == atol(fail)) {
+    if ((fail && atol(fail) > 0 && n == atol(fail)) || (above && atoll(above) > 0 && bytes > (size_t)atoll(above))) {
         set_error("device allocation failed (FSH_FAIL_ALLOC)");
         return E_NOMEM;
     }

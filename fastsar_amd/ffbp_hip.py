@@ -11,6 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _hip
+from . import memory as _mem
 from .plan import C, HOST_LEVELS, level_phase_constants, tile_digits
 
 STAGES = ('prepare', 'rot_fir_k', 'fir_p', 'device_phases', 'final_geom', 'final_tiles', 'level_fused')
@@ -35,6 +36,8 @@ class HipFFBP:
         fin = plan['final']
         self.plan = plan
         self.nx, self.ny = plan['nx'], plan['ny']
+        self.G = levels[0]['C']
+        self._group_req, self._warned_group = 0, None
         self.P, self.K = levels[0]['P'], levels[0]['K']
         keep = []                                   # host arrays referenced by the descriptors until create returns
 
@@ -106,7 +109,9 @@ class HipFFBP:
         return wf[0], wf[1], gyr
 
     def set_group(self, ng):
+        """First-level children per group (0 or None: automatic, up to 8, halved while device memory lacks)."""
         _hip.check(_hip.lib().fsh_ffbp_set_group(self.h, int(ng or 0)), 'ffbp_set_group')
+        self._group_req = int(ng or 0)
 
     def info(self):
         ng = ctypes.c_int()
@@ -147,6 +152,11 @@ class HipFFBP:
         bad = ctypes.c_int(-1)
         _hip.check(_hip.lib().fsh_ffbp_form(self.h, _hip.ptr(S), int(on_dev), _hip.ptr(img), int(img_dev), ctypes.byref(bad)),
                    'ffbp_form')
+        g = self.info()['group']
+        if self._group_req == 0 and g < min(8, self.G) and g != self._warned_group:
+            self._warned_group = g
+            _mem.warn(f'hip: out of device memory for groups of {min(8, self.G)} first-level children; the former '
+                      f'continues with {g} per group')
         return img
 
     def __del__(self):

@@ -200,3 +200,37 @@ def test_image_vs_oracle(scene, nlev, T, kernel, monkeypatch):
     form.set_group(1)
     img1 = form(S)
     assert np.array_equal(img, img1)
+
+
+def _group_bytes(plan, ng):
+    """the largest level buffer of a group of ng first-level children on the two-pass path (bytes)"""
+    big, parents = 0, 1
+    for l, lv in enumerate(plan['levels']):
+        outs = parents * (ng if l == 0 else lv['C'])
+        big = max(big, outs * lv['P'] * lv['Ko'] * 8, outs * lv['Po'] * lv['Ko'] * 8)
+        parents = outs
+    return big
+
+
+def test_group_halving_on_allocation_failure(scene, monkeypatch):
+    """a device that cannot hold the buffers of 8 first-level children: the former halves the group until they
+    fit, warns once (MemoryWarning), and forms the same image bit for bit"""
+    from fastsar_amd.memory import MemoryWarning
+    monkeypatch.setenv('FSH_LEVEL_KERNEL', 'split')
+    col, S = scene
+    p = pl.make_plan(col, 128, 128, 0.5, 0.5, T=8, nlev=2, e1=(0.0, 1.0, 0.0), e2=(1.0, 0.0, 0.0))
+    coll = pl.collection_arrays(p, col.ant)
+    assert p['levels'][0]['C'] >= 8
+    want = make_ffbp_hip(p, coll)(S)
+    cap = _group_bytes(p, 2) + 1                   # groups of 2 fit, groups of 4 do not
+    assert _group_bytes(p, 4) > cap
+    monkeypatch.setenv('FSH_FAIL_ALLOC_ABOVE', str(cap))
+    form = make_ffbp_hip(p, coll)
+    with pytest.warns(MemoryWarning, match='continues with 2'):
+        img = form(S)
+    assert form.info()['group'] == 2
+    assert np.array_equal(img, want)
+    # a requested group that does not fit is an error, not a fallback
+    form.set_group(8)
+    with pytest.raises(MemoryError):
+        form(S)

@@ -780,11 +780,18 @@ int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, con
         return E_INVALID;
     }
     // the matrix-core form where its taps fit (L <= 128) and the axis is decimated; FSH_ROT_KERNEL=valu|mfma forces one
+    // FSH_ROT_KERNEL=valu|mfma|mfma1 forces one (mfma: the row kernel where its taps fit; mfma1: the per-wave-row
+    // form); by default the row kernel for rows of four or more M-tiles, the per-wave-row form below that
     const char* force = getenv("FSH_ROT_KERNEL");
     const bool want_valu = force && strcmp(force, "valu") == 0;
-    const bool want_mfma = force && strcmp(force, "mfma") == 0;
+    const bool want_mfma1 = force && strcmp(force, "mfma1") == 0;
+    const bool want_mfma = (force && strcmp(force, "mfma") == 0) || want_mfma1;
     if (!want_valu && (D > 1 || want_mfma) && L <= 128) {
-        const int st = launch_rot_fir_k_mfma(x, Np, P, K, c0, sl, Cn, taps, L, pl, D, Ko, kc, y, s);
+        int st = E_INVALID;
+        // the row form amortizes each row's taps over its M-tiles: it wins from four M-tiles (128 outputs) up
+        const bool row_form = force && strcmp(force, "mfma") == 0 ? true : (Ko + 31) / 32 >= 4;
+        if (!want_mfma1 && row_form) st = launch_rot_fir_k_mf2(x, Np, P, K, c0, sl, Cn, taps, L, pl, D, Ko, kc, y, s);
+        if (st == E_INVALID) st = launch_rot_fir_k_mfma(x, Np, P, K, c0, sl, Cn, taps, L, pl, D, Ko, kc, y, s);
         if (st != E_INVALID) return st;
     }
     const int PB = 8;

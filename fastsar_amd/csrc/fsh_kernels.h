@@ -9,6 +9,9 @@ int launch_prepare(const float2* S, int P, int K, const float* wp, const float* 
 int launch_fill_int(int* p, int n, int value, hipStream_t s);
 int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
                      int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s);
+// the row form of the matrix-core range filter (range_mfma.hip); E_INVALID when the taps do not fit it
+int launch_rot_fir_k_mf2(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
+                         int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s);
 int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float2* z,
                  hipStream_t s);
 // the rotation and both filters of a level in one pass, for short children (fused_kernels.hip); hk_t and hp_t are the

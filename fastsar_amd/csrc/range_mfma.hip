@@ -1,0 +1,254 @@
+// Rotation + range decimation on the matrix cores, one workgroup per parent row (the stage of
+// fastsar/ffbp_cpu.cpp:147-255). With k_j = D j - pl and the child's ramp factored at k_j,
+//   y_c[j] = rho_c(j) sum_r x[k_j + r] g_c[r],   g_c[r] = h[r] exp(2 pi i sl_c r),
+//   rho_c(j) = exp(2 pi i (c0_c + (k_j - kc) sl_c)),
+// a real product Y [Ko x 2C] = X [Ko x 2K'] G [2K' x 2C] per row, (Re, Im) interleaved along the contraction, on
+// v_mfma_f32_32x32x16_bf16 with operands split into bfloat16 pairs (hi hi + hi lo + lo hi: float32-class sums).
+//
+// The contraction runs over the taps in polyphase order: tap r = D u + ph is contraction index kappa = ph NWp + u
+// (NWp = ceil(L / D) rounded up to even; indices past the taps carry zero). Then X[j, kappa] = x[D (j + u) + ph - pl]
+// is word j + u of plane ph, where plane ph holds x[D m + ph - pl] (m = 0, 1, ...): a lane (output j = its row of
+// the tile, half q) reads its four contraction indices 8s + 4q .. 8s + 4q + 3 as two pairs of adjacent words, and
+// the 32 lanes of a half read 32 consecutive words: no bank conflicts, two ds_read2_b32 per operand.
+//
+// Work: block (row p, parent n, chunk of M-tiles), 256 threads. The row's window is staged once into LDS as packed
+// bfloat16 (Re, Im) words, hi and lo planes; wave w owns N-tiles (16 children) w, w + 4, ... (when there are fewer
+// than four, the waves share an N-tile and split its M-tiles), builds that N-tile's complex taps in registers (exact
+// float64 seed per pair of taps) and runs every M-tile of the chunk against them. Results are rotated by rho and
+// leave through a per-wave LDS transpose so that each half-wave stores 32 consecutive outputs of one child.
+#include "fsh_common.h"
+#include "fsh_kernels.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+
+namespace fsh {
+
+namespace {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx16_t __attribute__((ext_vector_type(16)));
+typedef unsigned int uint4_t __attribute__((ext_vector_type(4)));
+
+constexpr int STG = 16 * 66;               // per-wave transpose buffer (floats): 16 children x 32 outputs x (Re, Im)
+
+__device__ __forceinline__ unsigned int pack2(__bf16 lo, __bf16 hi)
+{
+    return (unsigned int)__builtin_bit_cast(unsigned short, lo) | ((unsigned int)__builtin_bit_cast(unsigned short, hi) << 16);
+}
+
+__device__ __forceinline__ void split(float v, __bf16& hi, __bf16& lo)
+{
+    hi = (__bf16)v;
+    lo = (__bf16)(v - (float)hi);
+}
+
+// one pair of contraction indices (kappa, kappa + 1) = taps (r0, r0 + D) of phase ph: their values and the word
+// offset ph PS + u of their first sample in the window planes
+struct PairTab {
+    float h0, h1, r0;
+    int off;
+};
+
+}  // namespace
+
+template <int KS>
+__global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restrict__ x, int P, int K, const float* __restrict__ c0,
+                                                        const float* __restrict__ sl, int Cn, const float* __restrict__ taps,
+                                                        int L, int pl, int D, int Ko, double kc, int NWp, int NPH, int PS,
+                                                        int MC, float2* __restrict__ y)
+{
+    extern __shared__ unsigned int lds_u[];
+    unsigned int* XH = lds_u;                                  // [NPH][PS]
+    unsigned int* XL = XH + NPH * PS;
+    float* stg_all = reinterpret_cast<float*>(XL + NPH * PS);  // [4][STG]
+    PairTab* tab = reinterpret_cast<PairTab*>(stg_all + 4 * STG);   // [4 KS]
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int col = lane & 31, q = lane >> 5;
+    const int p = blockIdx.x, n = blockIdx.y;
+    const int MT = (Ko + 31) / 32;
+    const int m0 = blockIdx.z * MC, m1 = min(m0 + MC, MT);
+    const int WM = 32 * (m1 - m0) + NWp;                       // words per plane used by this chunk
+    const int ks = D * 32 * m0 - pl;                           // sample of word 0 of plane 0
+
+    for (int i = tid; i < 4 * KS; i += 256) {
+        const int kap = 2 * i, ph = kap / NWp, u = kap - ph * NWp;
+        const int r0 = D * u + ph;
+        PairTab e;
+        e.h0 = (ph < D && r0 < L) ? taps[r0] : 0.f;
+        e.h1 = (ph < D && r0 + D < L) ? taps[r0 + D] : 0.f;
+        e.r0 = (float)r0;
+        e.off = ph * PS + u;
+        tab[i] = e;
+    }
+    const float2* xrow = x + ((size_t)n * P + p) * K;
+    for (int i = tid; i < NPH * WM; i += 256) {
+        const int ph = i / WM, m = i - ph * WM;
+        const int k = ks + D * m + ph;
+        const float2 v = (ph < D && k >= 0 && k < K) ? xrow[k] : make_float2(0.f, 0.f);
+        __bf16 rh, rl, ih, il;
+        split(v.x, rh, rl);
+        split(v.y, ih, il);
+        XH[ph * PS + m] = pack2(rh, ih);
+        XL[ph * PS + m] = pack2(rl, il);
+    }
+    __syncthreads();
+
+    float* stg = stg_all + wv * STG;
+    const int NT = (2 * Cn + 31) / 32;
+    const int wpn = NT >= 4 ? 1 : 4 / NT;                      // waves per N-tile
+    const int nt_first = wv / wpn, ws = wv - nt_first * wpn;
+    const int nt_step = 4 / wpn;
+    int off[KS][2];
+    #pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        off[s][0] = tab[4 * s + 2 * q].off;
+        off[s][1] = tab[4 * s + 2 * q + 1].off;
+    }
+    for (int nt = nt_first; nt < NT; nt += nt_step) {
+        const int ncol = nt * 32 + col;
+        const int c = ncol >> 1, t = ncol & 1;
+        const bool cok = c < Cn;
+        double a = 0.0, b = 0.0;
+        if (cok) {
+            a = (double)c0[((size_t)n * Cn + c) * P + p];
+            b = (double)sl[((size_t)n * Cn + c) * P + p];
+        }
+        // the N-tile's taps: B[kappa][column] for this lane's column (child c, Re or Im) and contraction indices
+        // 8s + 4q .. 8s + 4q + 3; rows (Re x, Im x) of column Re y_c hold (Re g, -Im g), of column Im y_c (Im g, Re g)
+        bf16x8_t Gh[KS], Gl[KS];
+        {
+            float vdr, vdi;
+            cis_cyc(b * (double)D, vdr, vdi);
+            const float slf = cok ? sl[((size_t)n * Cn + c) * P + p] : 0.f;
+            int tb = 4 * 0;
+            asm volatile("" : "+s"(tb));                       // the table reads stay here, not hoisted
+            #pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                #pragma unroll
+                for (int pr = 0; pr < 2; ++pr) {
+                    const PairTab e = tab[tb + 4 * s + 2 * q + pr];
+                    // exp(2 pi i sl r0): the product's rounding error recovered by an fma, the cycle wrapped exactly
+                    const float pp = slf * e.r0, pe = fmaf(slf, e.r0, -pp);
+                    const float fr = (pp - rintf(pp)) + pe;
+                    const float sr = __builtin_amdgcn_cosf(fr), si = __builtin_amdgcn_sinf(fr);
+                    const float s2r = fmaf(sr, vdr, -(si * vdi)), s2i = fmaf(sr, vdi, si * vdr);
+                    const float h0 = cok ? e.h0 : 0.f, h1 = cok ? e.h1 : 0.f;
+                    const float g0r = h0 * sr, g0i = h0 * si, g1r = h1 * s2r, g1i = h1 * s2i;
+                    const float a0 = t ? g0i : g0r, b0 = t ? g0r : -g0i;
+                    const float a1 = t ? g1i : g1r, b1 = t ? g1r : -g1i;
+                    __bf16 hh, ll;
+                    split(a0, hh, ll); Gh[s][4 * pr + 0] = hh; Gl[s][4 * pr + 0] = ll;
+                    split(b0, hh, ll); Gh[s][4 * pr + 1] = hh; Gl[s][4 * pr + 1] = ll;
+                    split(a1, hh, ll); Gh[s][4 * pr + 2] = hh; Gl[s][4 * pr + 2] = ll;
+                    split(b1, hh, ll); Gh[s][4 * pr + 3] = hh; Gl[s][4 * pr + 3] = ll;
+                }
+            }
+        }
+        float sDr, sDi, s8r, s8i;
+        cis_cyc(b * (double)D, sDr, sDi);
+        cis_cyc(b * (double)(8 * D), s8r, s8i);
+        for (int mt = m0 + ws; mt < m1; mt += wpn) {
+            floatx16_t acc = {};
+            const int jb = (mt - m0) * 32 + col;
+            #pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const unsigned int* h0 = XH + off[s][0] + jb;
+                const unsigned int* h1 = XH + off[s][1] + jb;
+                const unsigned int* l0 = XL + off[s][0] + jb;
+                const unsigned int* l1 = XL + off[s][1] + jb;
+                uint4_t wh, wl;
+                wh.x = h0[0]; wh.y = h0[1]; wh.z = h1[0]; wh.w = h1[1];
+                wl.x = l0[0]; wl.y = l0[1]; wl.z = l1[0]; wl.w = l1[1];
+                const bf16x8_t Ah = __builtin_bit_cast(bf16x8_t, wh), Al = __builtin_bit_cast(bf16x8_t, wl);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gh[s], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gl[s], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Gh[s], acc, 0, 0, 0);
+            }
+            // rho at rows 4q, 4q + 1, ... of this tile: seeded at row 4q, stepped by D samples, 8 rows apart by 8 D;
+            // the lane pair (t = 0, 1) holds (Re, Im) of one output: Re' = Re er - Im ei, Im' = Im er + Re ei
+            float rr, ri;
+            cis_cyc(fma((double)(D * (mt * 32 + 4 * q) - pl) - kc, b, a), rr, ri);
+            #pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                float er = rr, ei = ri;
+                #pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int i = 4 * g + e;
+                    const float mine = acc[i];
+                    const float other = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, mine), 0xB1, 0xF, 0xF, false));
+                    stg[(col >> 1) * 66 + 2 * (8 * g + 4 * q + e) + t] = fmaf(mine, er, other * (t ? ei : -ei));
+                    const float nr = fmaf(er, sDr, -(ei * sDi));
+                    ei = fmaf(er, sDi, ei * sDr);
+                    er = nr;
+                }
+                const float nr = fmaf(rr, s8r, -(ri * s8i));
+                ri = fmaf(rr, s8i, ri * s8r);
+                rr = nr;
+            }
+            wave_lds_sync();
+            const int j = mt * 32 + col;
+            #pragma unroll
+            for (int it = 0; it < 8; ++it) {
+                const int cl = q + 2 * it, cc = nt * 16 + cl;
+                if (cc < Cn && j < Ko) {
+                    const float2 v = *reinterpret_cast<const float2*>(stg + cl * 66 + 2 * col);
+                    y[(((size_t)n * Cn + cc) * P + p) * Ko + j] = v;
+                }
+            }
+            wave_lds_sync();                                   // the next tile's writes to stg come after these reads
+        }
+    }
+}
+
+// launch shape of the row kernel: contraction steps KS, window planes NPH of stride PS, M-tiles per block MC, LDS
+// bytes; false when the taps do not fit (D ceil(L / D) rounded to even > 128)
+static bool mf2_shape(int L, int D, int Ko, int& NWp, int& KS, int& NPH, int& PS, int& MC, int& chunks, size_t& lds)
+{
+    const int NW = (L + D - 1) / D;
+    NWp = NW + (NW & 1);
+    KS = (D * NWp + 7) / 8;
+    if (KS < 1 || KS > 16) return false;
+    NPH = (8 * KS + NWp - 1) / NWp;
+    const int MT = (Ko + 31) / 32;
+    // window planes within ~36 KB so that three blocks share a CU (with the transpose buffers and the tap table)
+    const size_t fixed = (size_t)4 * 4 * STG + sizeof(PairTab) * 4 * KS;
+    const size_t budget = 52 * 1024 - fixed;
+    MC = std::max(1, (int)((budget / (8 * (size_t)NPH) - NWp - 1) / 32));
+    MC = std::min(MC, MT);
+    chunks = (MT + MC - 1) / MC;
+    MC = (MT + chunks - 1) / chunks;                           // even chunks
+    PS = 32 * MC + NWp + 1;
+    lds = (size_t)8 * NPH * PS + fixed;
+    return lds <= 160 * 1024;
+}
+
+int launch_rot_fir_k_mf2(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
+                         int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s)
+{
+    int NWp, KS, NPH, PS, MC, chunks;
+    size_t lds;
+    if (!mf2_shape(L, D, Ko, NWp, KS, NPH, PS, MC, chunks, lds)) return E_INVALID;
+    for (int n0 = 0; n0 < Np; n0 += 65535) {
+        const int nn = std::min(65535, Np - n0);
+        const dim3 grid((unsigned)P, (unsigned)nn, (unsigned)chunks);
+        const float2* xs = x + (size_t)n0 * P * K;
+        const float* c0s = c0 + (size_t)n0 * Cn * P;
+        const float* sls = sl + (size_t)n0 * Cn * P;
+        float2* ys = y + (size_t)n0 * Cn * P * Ko;
+#define FSH_M2(KK) hipLaunchKernelGGL(k_rot_fir_k_mf2<KK>, grid, dim3(256), lds, s, xs, P, K, c0s, sls, Cn, taps, L, pl, D, Ko, \
+                                      kc, NWp, NPH, PS, MC, ys)
+        switch (KS) {
+            case 1: FSH_M2(1); break;   case 2: FSH_M2(2); break;   case 3: FSH_M2(3); break;   case 4: FSH_M2(4); break;
+            case 5: FSH_M2(5); break;   case 6: FSH_M2(6); break;   case 7: FSH_M2(7); break;   case 8: FSH_M2(8); break;
+            case 9: FSH_M2(9); break;   case 10: FSH_M2(10); break; case 11: FSH_M2(11); break; case 12: FSH_M2(12); break;
+            case 13: FSH_M2(13); break; case 14: FSH_M2(14); break; case 15: FSH_M2(15); break; default: FSH_M2(16); break;
+        }
+#undef FSH_M2
+        FSH_TRY(hipGetLastError());
+    }
+    return OK;
+}
+
+}  // namespace fsh

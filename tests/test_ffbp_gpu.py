@@ -23,10 +23,11 @@ ROT_CASES = [(3, 23, 5, 2, 301), (9, 67, 3, 1, 301), (4, 31, 2, 3, 301), (1, 1, 
              (3, 23, 17, 1, 4001), (8, 60, 24, 3, 313), (2, 18, 6, 2, 196)]
 
 
-@pytest.mark.parametrize('kernel', ['valu', 'mfma'])
+@pytest.mark.parametrize('kernel', ['valu', 'mfma', 'mfma1'])
 @pytest.mark.parametrize('D,L,Cn,Np,K', ROT_CASES)
 def test_rot_fir_k_stage(D, L, Cn, Np, K, kernel, monkeypatch):
-    """rotation + range decimation on both kernels (VALU float32; MFMA bfloat16 x 3) against the float64 oracle"""
+    """rotation + range decimation on every kernel (VALU float32; the MFMA row kernel and the per-wave-row MFMA
+    kernel, bfloat16 x 3) against the float64 oracle"""
     monkeypatch.setenv('FSH_ROT_KERNEL', kernel)
     rng = np.random.default_rng(D * 100 + L + Cn)
     P = 37
@@ -222,10 +223,13 @@ def test_group_halving_on_allocation_failure(scene, monkeypatch):
     coll = pl.collection_arrays(p, col.ant)
     assert p['levels'][0]['C'] >= 8
     want = make_ffbp_hip(p, coll)(S)
+    form = make_ffbp_hip(p, coll)
+    form.set_group(1)
+    form(S)                                        # the history's upload buffer exists from here on
+    form.set_group(0)
     cap = _group_bytes(p, 2) + 1                   # groups of 2 fit, groups of 4 do not
     assert _group_bytes(p, 4) > cap
     monkeypatch.setenv('FSH_FAIL_ALLOC_ABOVE', str(cap))
-    form = make_ffbp_hip(p, coll)
     with pytest.warns(MemoryWarning, match='continues with 2'):
         img = form(S)
     assert form.info()['group'] == 2

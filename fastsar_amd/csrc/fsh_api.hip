@@ -105,6 +105,7 @@ struct Level {
     float* hk_t = nullptr;         // polyphase tables of the fused form (fused levels only)
     float* hp_t = nullptr;
     bool fused = false;            // rotation and both filters in one pass (launch_level_fused)
+    bool mfma = false;             // the same on the matrix cores (launch_level_mfma)
     long long below = 1;           // tiles per parent... children per first-level child at this level's input
 };
 
@@ -183,8 +184,9 @@ struct Former {
             const fsh_level_desc& h = lv[l].h;
             const long long ch = (l == 0) ? g : (long long)h.C;
             const long long outs = parents * ch;
-            if (!lv[l].fused) maxbuf = std::max(maxbuf, (size_t)(outs * h.P * h.Ko));
-            if (h.Dp > 1 || lv[l].fused) maxbuf = std::max(maxbuf, (size_t)(outs * h.Po * h.Ko));
+            const bool one_pass = lv[l].fused || lv[l].mfma;
+            if (!one_pass) maxbuf = std::max(maxbuf, (size_t)(outs * h.P * h.Ko));
+            if (h.Dp > 1 || one_pass) maxbuf = std::max(maxbuf, (size_t)(outs * h.Po * h.Ko));
             if (l >= 2) maxph = std::max(maxph, (size_t)(parents * h.C * h.P));
             parents = outs;
         }
@@ -256,6 +258,18 @@ struct Former {
                 mark_end(ST_PHASE);
                 c0 = c0b;
                 sl = slb;
+            }
+            if (L.mfma) {
+                mark_begin();
+                if ((st = launch_level_mfma(cur, (int)parents, h.P, h.K, c0, sl, ch, L.taps_k, h.Lk, h.Dk, h.plk, h.Ko, L.hp_t,
+                                            h.Lp, h.Dp, h.plp, h.Po, h.kc, buf[yb], stream)))
+                    return st;
+                mark_end(ST_LEVEL);
+                cur = buf[yb];
+                curbuf = yb;
+                first_parent = (l == 0) ? g0 : first_parent * h.C;
+                parents *= ch;
+                continue;
             }
             if (L.fused) {
                 mark_begin();
@@ -391,6 +405,33 @@ int fsh_level_fused(const float* x, int Np, int P, int K, const float* c0, const
     return st;
 }
 
+int fsh_level_mfma(const float* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps_k,
+                   int Lk, int plk, int Dk, int Ko, const float* taps_p, int Lp, int plp, int Dp, int Po, double kc, float* z,
+                   void* stream)
+{
+    if (!x || !c0 || !sl || !z || !taps_k || !taps_p) { set_error("level_mfma: null argument"); return E_INVALID; }
+    int KS, NWp, NPH, PS, JT, njr, nseg, IS, nwp;
+    size_t lds;
+    if (Np < 1 || Lk < 1 || Lp < 1 || !level_mfma_shape(Np, Cn, Lk, Dk, Ko, Lp, Dp, Po, KS, NWp, NPH, PS, JT, njr, nseg, IS,
+                                                        lds, nwp)) {
+        set_error("level_mfma: the level does not fit the fused matrix-core form (Dk, Dp >= 2, Ko >= 16, Dk ceil(Lk / Dk) "
+                  "<= 128, ceil(Lp / Dp) <= 16)");
+        return E_INVALID;
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    float* t = nullptr;
+    int st = dev_alloc((void**)&t, (size_t)Dp * nwp * sizeof(float));
+    if (st) return st;
+    if (!(st = launch_polyphase_nw(taps_p, Lp, Dp, nwp, t, s)) &&
+        !(st = launch_level_mfma((const float2*)x, Np, P, K, c0, sl, Cn, taps_k, Lk, Dk, plk, Ko, t, Lp, Dp, plp, Po, kc,
+                                 (float2*)z, s))) {
+        const hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_status(e, "level_mfma");
+    }
+    dev_free(t);
+    return st;
+}
+
 int fsh_device_phases(const double* refs, const double* u, const double* r0, const double* d, int Np, int Cn, int P,
                       double k0c, double k1, float* c0, float* sl, void* stream)
 {
@@ -473,6 +514,12 @@ int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, voi
                     (st = launch_polyphase(h.Dp > 1 ? L.taps_p : nullptr, Lp, h.Dp, L.hp_t, f->stream)))
                     return fail(st);
                 L.fused = true;
+            } else if (l > 0 && level_mfma_preferred(h.C, Lk, h.Dk, h.Ko, Lp, h.Dp, h.Po)) {
+                const int nw = level_mfma_nw(Lp, h.Dp);
+                if ((st = dev_alloc((void**)&L.hp_t, (size_t)h.Dp * nw * sizeof(float))) ||
+                    (st = launch_polyphase_nw(L.taps_p, Lp, h.Dp, nw, L.hp_t, f->stream)))
+                    return fail(st);
+                L.mfma = true;
             }
         }
         L.below = children;

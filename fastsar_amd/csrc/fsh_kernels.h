@@ -20,6 +20,16 @@ int fused_nw(int L, int D);
 bool level_fused_shape(int C, int P, int K, int Lk, int Dk, int Ko, int Lp, int Dp, int& CB, int& R, size_t& lds);
 bool level_fused_preferred(int C, int P, int K, int Lk, int Dk, int Ko, int Lp, int Dp);
 int launch_polyphase(const float* h, int L, int D, float* t, hipStream_t s);
+int launch_polyphase_nw(const float* h, int L, int D, int NW, float* t, hipStream_t s);
+// a level on the matrix cores with the pulse filter fused (level_mfma.hip): both axes decimated, hp_t the pulse
+// filter's polyphase table for level_mfma_nw(Lp, Dp) accumulators
+int level_mfma_nw(int Lp, int Dp);
+bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int Po, int& KS, int& NWp, int& NPH, int& PS,
+                      int& JT, int& njr, int& nseg, int& IS, size_t& lds, int& nwp);
+bool level_mfma_preferred(int C, int Lk, int Dk, int Ko, int Lp, int Dp, int Po);
+int launch_level_mfma(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int C, const float* hk,
+                      int Lk, int Dk, int plk, int Ko, const float* hp_t, int Lp, int Dp, int plp, int Po, double kc,
+                      float2* z, hipStream_t s);
 int launch_level_fused(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int C, const float* hk_t,
                        int Lk, int Dk, int plk, int Ko, const float* hp_t, int Lp, int Dp, int plp, int Po, double kc,
                        float2* z, hipStream_t s);

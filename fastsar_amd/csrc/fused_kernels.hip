@@ -221,8 +221,12 @@ __global__ void k_polyphase(const float* __restrict__ h, int L, int D, int NW, f
 
 int launch_polyphase(const float* h, int L, int D, float* t, hipStream_t s)
 {
-    const int NW = fused_nw(L, D);
-    if (D < 1 || L < 1 || NW < 0) return E_INVALID;
+    return launch_polyphase_nw(h, L, D, fused_nw(L, D), t, s);
+}
+
+int launch_polyphase_nw(const float* h, int L, int D, int NW, float* t, hipStream_t s)
+{
+    if (D < 1 || L < 1 || NW < 1 || NW * D < L) return E_INVALID;
     hipLaunchKernelGGL(k_polyphase, dim3(1), dim3(256), 0, s, h, L, D, NW, t);
     FSH_TRY(hipGetLastError());
     return OK;

@@ -74,6 +74,14 @@ int fsh_level_fused(const float* x, int Np, int P, int K, const float* c0, const
                     int Lk, int plk, int Dk, int Ko, const float* taps_p, int Lp, int plp, int Dp, int Po, double kc, float* z,
                     void* stream);
 
+/* The same on the matrix cores (bfloat16 x 3 range filter, float32 pulse filter in registers), for levels with
+ * both axes decimated: Dk, Dp >= 2, Ko >= 16, Dk ceil(Lk / Dk) <= 128 and ceil(Lp / Dp) <= 16 (FSH_EINVAL otherwise).
+ * Arguments as fsh_level_fused; synchronizes the stream. Replaces the pair rot_fir_k_cplx + fir_p of a level
+ * (fastsar/ffbp_cpu.cpp:244-293). */
+int fsh_level_mfma(const float* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps_k,
+                   int Lk, int plk, int Dk, int Ko, const float* taps_p, int Lp, int plp, int Dp, int Po, double kc, float* z,
+                   void* stream);
+
 /* Child rotations of a level computed on the device in float64: refs [Np, 3] parent centres, u [P, 3], r0 [P] the
  * level's track, d [Cn, 3] child offsets -> c0, sl [Np, Cn, P] float32. Replaces the device-phase kernel and the
  * host form (fastsar/ffbp_cuda.py:598-632, fastsar/ffbp_cpu.py:142-155). */

@@ -31,7 +31,7 @@ def test_header_declares_the_abi():
     names = _declared()
     assert names == _hip.exported_symbols()
     for must in ('fsh_ffbp_create', 'fsh_ffbp_form', 'fsh_ffbp_destroy', 'fsh_exact_create', 'fsh_exact_form',
-                 'fsh_exact_destroy', 'fsh_rot_fir_k', 'fsh_fir_p', 'fsh_final_tiles', 'fsh_level_fused',
+                 'fsh_exact_destroy', 'fsh_rot_fir_k', 'fsh_fir_p', 'fsh_final_tiles', 'fsh_level_fused', 'fsh_level_mfma',
                  'fsh_last_error', 'fsh_version', 'fsh_device_info'):
         assert must in names
 
@@ -67,6 +67,7 @@ def test_stage_shape_validation(lib):
     assert lib.fsh_rot_fir_k(vp, 0, 10, 10, vp, vp, 1, vp, 1, 0, 1, 10, 4.5, vp, vp) == _hip.EINVAL
     assert lib.fsh_fir_p(vp, 1, 10, 10, vp, 3, -1, 2, 5, vp, vp) == _hip.EINVAL
     assert lib.fsh_level_fused(vp, 1, 10, 20, vp, vp, 1, vp, 25, 12, 5, 8, vp, 3, 1, 2, 6, 9.5, vp, vp) == _hip.EINVAL
+    assert lib.fsh_level_mfma(vp, 1, 10, 20, vp, vp, 1, vp, 25, 12, 5, 8, vp, 3, 1, 2, 6, 9.5, vp, vp) == _hip.EINVAL
     assert lib.fsh_final_tiles(vp, 1, 4, 4, 12, vp, vp, vp, 1.0, 1.0, 0.0, 0.0, 0.0, vp, vp, vp, vp, 0, 0, 12, 12, vp,
                                vp) == _hip.EINVAL
 

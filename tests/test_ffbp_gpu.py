@@ -104,6 +104,36 @@ def test_level_fused_stage(Dk, pk, Dp, pp, Cn, Np, P, K):
     assert e < -115, e
 
 
+MFMA_LEVEL_CASES = [(9, 0.6, 11, 0.6, 64, 1, 300, 600), (8, 0.6, 8, 0.6, 64, 2, 200, 315), (3, 0.5, 4, 0.7, 13, 2, 150, 200),
+                    (4, 0.6, 4, 0.6, 8, 1, 2000, 100), (2, 0.6, 2, 0.6, 8, 1, 100, 700), (3, 0.4, 6, 0.6, 24, 3, 260, 90)]
+
+
+@pytest.mark.parametrize('Dk,pk,Dp,pp,Cn,Np,P,K', MFMA_LEVEL_CASES)
+def test_level_mfma_stage(Dk, pk, Dp, pp, Cn, Np, P, K):
+    """a level on the matrix cores with the pulse filter fused (bfloat16 x 3 range filter, float32 pulse filter)
+    against the float64 oracle's two stages"""
+    rng = np.random.default_rng(Dk * 1000 + Dp * 100 + Cn + K)
+    x = (rng.standard_normal((Np, P, K)) + 1j * rng.standard_normal((Np, P, K))).astype(np.complex64)
+    c0 = rng.uniform(-0.5, 0.5, (Np, Cn, P)).astype(np.float32)
+    sl = rng.uniform(-0.8, 0.8, (Np, Cn, P)).astype(np.float32)
+    frk, _ = pl.decimator_fir(K, Dk, pk)
+    frp, _ = pl.decimator_fir(P, Dp, pp)
+    Ko, Po, kc = frk['n_out'], frp['n_out'], (K - 1) / 2.0
+    yre, yim = oracle.rot_fir_k(x.real, x.imag, c0, sl, frk, Dk, Ko, kc)
+    zre, zim = oracle.fir_p(yre, yim, frp, Dp, Po)
+    dx, dc, ds = (_hip.DeviceArray.from_host(a) for a in (x, c0, sl))
+    dtk, dtp = (_hip.DeviceArray.from_host(f['kern'].astype(np.float32)) for f in (frk, frp))
+    dz = _hip.DeviceArray((Np * Cn, Po, Ko), np.complex64)
+    print(f'level_mfma Dk={Dk} Lk={frk["L"]} Dp={Dp} Lp={frp["L"]} C={Cn} Np={Np} P={P} K={K} -> [{Po} x {Ko}]', flush=True)
+    _hip.check(_hip.lib().fsh_level_mfma(_hip.ptr(dx), Np, P, K, _hip.ptr(dc), _hip.ptr(ds), Cn, _hip.ptr(dtk), frk['L'],
+                                         frk['pl'], Dk, Ko, _hip.ptr(dtp), frp['L'], frp['pl'], Dp, Po, kc, _hip.ptr(dz), None),
+               'level_mfma')
+    e = db(dz.to_host(), zre + 1j * zim)
+    print(f'  {e:.1f} dB')
+    # bfloat16 x 3 leaves out the lo x lo product, 2^-16 of each term: about -96 dB on long filters
+    assert e < -94, e
+
+
 def test_level_fused_rejects():
     """shapes outside the fused form are refused before any launch"""
     lib = _hip.lib()
@@ -180,13 +210,17 @@ def scene():
     return col, sim.simulate_brute(col, pos, amp).astype(np.complex64)
 
 
-@pytest.mark.parametrize('kernel', ['valu', 'mfma', 'fused'])
+@pytest.mark.parametrize('kernel', ['valu', 'mfma', 'fused', 'auto', 'lmfma'])
 @pytest.mark.parametrize('nlev,T', [(2, 16), (3, 16), (3, 8), (2, 32), (4, 4)])
 def test_image_vs_oracle(scene, nlev, T, kernel, monkeypatch):
-    """whole images (float32 VALU levels, bfloat16 x 3 MFMA levels, or MFMA level 0 and fused later levels) against
-    the float64 oracle's image"""
-    monkeypatch.setenv('FSH_ROT_KERNEL', 'valu' if kernel == 'valu' else 'mfma')
-    monkeypatch.setenv('FSH_LEVEL_KERNEL', 'fused' if kernel == 'fused' else 'split')
+    """whole images against the float64 oracle's image: float32 VALU levels, bfloat16 x 3 MFMA levels, MFMA level 0
+    with the later levels fused on VALU where they fit, the default choice, and the later levels fused on the matrix
+    cores where they fit"""
+    if kernel == 'lmfma':
+        monkeypatch.setenv('FSH_LEVEL_KERNEL', 'mfma')
+    elif kernel != 'auto':
+        monkeypatch.setenv('FSH_ROT_KERNEL', 'valu' if kernel == 'valu' else 'mfma')
+        monkeypatch.setenv('FSH_LEVEL_KERNEL', 'fused' if kernel == 'fused' else 'split')
     col, S = scene
     e1, e2 = (0.0, 1.0, 0.0), (1.0, 0.0, 0.0)
     p = pl.make_plan(col, 128, 128, 0.5, 0.5, T=T, nlev=nlev, e1=e1, e2=e2)

@@ -505,7 +505,8 @@ int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, voi
                 return fail(st);
         }
         {
-            // later levels take the fused form where it applies (level 0's children come in groups)
+            // later levels take a one-pass form where one applies (level 0's children come in groups): on VALU for
+            // short children whose parent one block holds, else on the matrix cores, else the two-pass form
             const int Lk = h.Dk > 1 ? h.Lk : 1, Lp = h.Dp > 1 ? h.Lp : 1;
             if (l > 0 && level_fused_preferred(h.C, h.P, h.K, Lk, h.Dk, h.Ko, Lp, h.Dp)) {
                 if ((st = dev_alloc((void**)&L.hk_t, (size_t)h.Dk * fused_nw(Lk, h.Dk) * sizeof(float))) ||

@@ -252,7 +252,8 @@ bool level_fused_shape(int C, int P, int K, int Lk, int Dk, int Ko, int Lp, int 
 
 // whether a former takes the fused form for a level: where one workgroup holds all of a parent's children (the
 // parent's rows are read once) and they cover at least half of its threads; the two-pass form (the matrix-core range
-// filter) where the children are long or many. FSH_LEVEL_KERNEL=fused|split forces either where it applies.
+// filter) where the children are long or many. FSH_LEVEL_KERNEL=fused forces it where it applies, =split or =mfma
+// turns it off.
 bool level_fused_preferred(int C, int P, int K, int Lk, int Dk, int Ko, int Lp, int Dp)
 {
     int CB, R;
@@ -260,6 +261,7 @@ bool level_fused_preferred(int C, int P, int K, int Lk, int Dk, int Ko, int Lp, 
     if (!level_fused_shape(C, P, K, Lk, Dk, Ko, Lp, Dp, CB, R, lds)) return false;
     const char* force = getenv("FSH_LEVEL_KERNEL");
     if (force && strcmp(force, "fused") == 0) return true;
+    if (force && strcmp(force, "mfma") == 0) return false;
     return CB == C && CB * Ko >= 128;
 }
 

@@ -56,6 +56,20 @@ __device__ __forceinline__ void cis_lin(float a, float b, float m, float& cs, fl
     sn = __builtin_amdgcn_sinf(fr);
 }
 
+// both halves of a bfloat16 pair split: hi = (bf16 a, bf16 b) packed, lo = the bf16 of the remainders
+__device__ __forceinline__ void split2(float a, float b, unsigned int& hi, unsigned int& lo)
+{
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    const bf16x2_t h = {(__bf16)a, (__bf16)b};
+    hi = __builtin_bit_cast(unsigned int, h);
+    const float ha = __builtin_bit_cast(float, hi << 16), hb = __builtin_bit_cast(float, hi & 0xffff0000u);
+    const bf16x2_t l = {(__bf16)(a - ha), (__bf16)(b - hb)};
+    lo = __builtin_bit_cast(unsigned int, l);
+}
+
+constexpr int MAXI = 4;                    // window words staged per thread and row (at most)
+constexpr int MAXG = 4;                    // tap entries built per thread and row (at most)
+
 struct TapK {                              // contraction index kappa: its tap value and tap number
     float h, r;
 };
@@ -105,49 +119,70 @@ __global__ __launch_bounds__(64 * JT_MAX) void k_level_mfma(
     __syncthreads();
 
     const float2* xn = x + (size_t)n * P * K;
-    // stage row p (planes and taps) into buffer b
-    auto stage = [&](int p, int b) {
+    // Staging, with the global loads of row p + 1 issued before row p computes and written to LDS after it:
+    //  * plane items idx = tid + k nthr (k < MAXI) of the NPH x WM window words: word m of plane ph, sample
+    //    Dk (jbase + m) + ph - plk (the same for every row);
+    //  * tap entries i = tid + k nthr (k < MAXG) of the 16 KS x CB (contraction index, child) table: child tid mod 8
+    //    for every k (nthr is a multiple of 8), so each thread reads one slope per row.
+    int it_lds[MAXI], it_k[MAXI];
+    #pragma unroll
+    for (int k = 0; k < MAXI; ++k) {
+        const int idx = tid + k * nthr;
+        const int ph = idx / WM, m = idx - ph * WM;
+        const int kk = Dk * (jbase + m) + ph - plk;
+        const bool ok = idx < NPH * WM;
+        it_lds[k] = ok ? ph * PS + m : -1;
+        it_k[k] = (ok && ph < Dk && kk >= 0 && kk < K) ? kk : -1;
+    }
+    const int gc = tid & 7, gcg = cb * CB + gc;
+    float g_h[MAXG], g_r[MAXG];
+    int g_w0[MAXG], g_w1[MAXG];
+    #pragma unroll
+    for (int k = 0; k < MAXG; ++k) {
+        const int i = tid + k * nthr, kap = i >> 3;
+        const bool ok = kap < 16 * KS;
+        const TapK e = tabk[ok ? kap : 0];
+        g_h[k] = (ok && gcg < C) ? e.h : 0.f;
+        g_r[k] = e.r;
+        const int s = kap >> 4, g = (kap >> 2) & 3, ew = kap & 3, L0 = 16 * g + 2 * gc;
+        g_w0[k] = ok ? ((s * 64 + L0) * 2) * 4 + ew : -1;      // word of the hi uint4; +4: the lo one
+        g_w1[k] = ok ? ((s * 64 + L0 + 1) * 2) * 4 + ew : -1;
+    }
+    float2 xv[MAXI];
+    float gsl = 0.f;
+    auto load_row = [&](int p) {
+        const float2* xr = xn + (size_t)p * K;
+        #pragma unroll
+        for (int k = 0; k < MAXI; ++k) xv[k] = it_k[k] >= 0 ? xr[it_k[k]] : make_float2(0.f, 0.f);
+        gsl = gcg < C ? sl[((size_t)n * C + gcg) * P + p] : 0.f;
+    };
+    auto store_row = [&](int b) {
         unsigned int* XH = planes + (2 * b) * NPH * PS;
         unsigned int* XL = XH + NPH * PS;
-        const float2* xr = xn + (size_t)p * K;
-        for (int ph = 0; ph < NPH; ++ph) {
-            for (int m = tid; m < WM; m += nthr) {
-                const int k = Dk * (jbase + m) + ph - plk;
-                const float2 v = (ph < Dk && k >= 0 && k < K) ? xr[k] : make_float2(0.f, 0.f);
-                __bf16 rh, rl, ih, il;
-                split(v.x, rh, rl);
-                split(v.y, ih, il);
-                XH[ph * PS + m] = pack2(rh, ih);
-                XL[ph * PS + m] = pack2(rl, il);
+        #pragma unroll
+        for (int k = 0; k < MAXI; ++k) {
+            if (it_lds[k] >= 0) {
+                unsigned int hi, lo;
+                split2(xv[k].x, xv[k].y, hi, lo);
+                XH[it_lds[k]] = hi;
+                XL[it_lds[k]] = lo;
             }
         }
-        // taps: entry (kappa, child) -> fragment lanes 16 g + 2c (column Re y_c) and + 1 (Im y_c), word e, where
-        // kappa = 16 s + 4 g + e; column Re y_c holds (Re g, -Im g), column Im y_c (Im g, Re g)
         unsigned int* FH = reinterpret_cast<unsigned int*>(frag + (size_t)b * KS * 64 * 2);
-        for (int i = tid; i < 16 * KS * CB; i += nthr) {
-            const int kap = i >> 3, c = i & 7, cg = cb * CB + c;
-            const TapK e = tabk[kap];
-            float gr = 0.f, gi = 0.f;
-            if (cg < C && e.h != 0.f) {
-                const float s = sl[((size_t)n * C + cg) * P + p];
+        #pragma unroll
+        for (int k = 0; k < MAXG; ++k) {
+            if (g_w0[k] >= 0) {
                 float cs, sn;
-                cis_lin(0.f, s, e.r, cs, sn);
-                gr = e.h * cs;
-                gi = e.h * sn;
+                cis_lin(0.f, gsl, g_r[k], cs, sn);
+                const float gr = g_h[k] * cs, gi = g_h[k] * sn;
+                unsigned int hi, lo;
+                split2(gr, -gi, hi, lo);                       // column Re y_c: (Re g, -Im g)
+                FH[g_w0[k]] = hi;
+                FH[g_w0[k] + 4] = lo;
+                // column Im y_c: (Im g, Re g), the same halves swapped with the sign of Im restored
+                FH[g_w1[k]] = __builtin_amdgcn_alignbit(hi, hi, 16) ^ 0x8000u;
+                FH[g_w1[k] + 4] = __builtin_amdgcn_alignbit(lo, lo, 16) ^ 0x8000u;
             }
-            __bf16 h1, l1, h2, l2, h3, l3;
-            split(gr, h1, l1);
-            split(gi, h2, l2);
-            split(-gi, h3, l3);
-            const int s = kap >> 4, g = (kap >> 2) & 3, ew = kap & 3;
-            const int L0 = 16 * g + 2 * c;
-            // uint4 per (s, lane): hi at [s][lane][0], lo at [s][lane][1]; word ew of each
-            unsigned int* w0 = FH + ((size_t)(s * 64 + L0) * 2) * 4 + ew;
-            unsigned int* w1 = FH + ((size_t)(s * 64 + L0 + 1) * 2) * 4 + ew;
-            w0[0] = pack2(h1, h3);
-            w0[4] = pack2(l1, l3);
-            w1[0] = pack2(h2, h1);
-            w1[4] = pack2(l2, l1);
         }
     };
 
@@ -195,11 +230,27 @@ __global__ __launch_bounds__(64 * JT_MAX) void k_level_mfma(
     // rows before 0 (first segment): zero rows, only the period bookkeeping
     for (int p = pbeg; p < plo; ++p)
         if (++pph == Dp) { end_period(); pph = 0; }
-    if (plo < phi) stage(plo, 0);
+    // the rotation constants of the row being computed (c0, slope of this lane's child), loaded a row ahead
+    float rc0 = 0.f, rsl = 0.f, nc0 = 0.f, nsl = 0.f;
+    auto load_rot = [&](int p, float& a, float& b) {
+        if (cok) {
+            a = c0[((size_t)n * C + cg) * P + p];
+            b = sl[((size_t)n * C + cg) * P + p];
+        }
+    };
+    if (plo < phi) {
+        load_row(plo);
+        load_rot(plo, rc0, rsl);
+        store_row(0);
+    }
     __syncthreads();
     for (int p = plo; p < phi; ++p) {
         const int b = (p - plo) & 1;
-        if (p + 1 < phi) stage(p + 1, b ^ 1);
+        const bool more = p + 1 < phi;
+        if (more) {
+            load_row(p + 1);
+            load_rot(p + 1, nc0, nsl);
+        }
         if (wact) {
             const unsigned int* XH = planes + (2 * b) * NPH * PS;
             const unsigned int* XL = XH + NPH * PS;
@@ -222,14 +273,9 @@ __global__ __launch_bounds__(64 * JT_MAX) void k_level_mfma(
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, Bh, acc, 0, 0, 0);
             }
             // rho_c(p, j) at the lane's outputs jout0 .. jout0 + 3: the lane pair (t = 0, 1) holds (Re, Im)
-            float a = 0.f, s = 0.f;
-            if (cok) {
-                a = c0[((size_t)n * C + cg) * P + p];
-                s = sl[((size_t)n * C + cg) * P + p];
-            }
             float er, ei, dr, di;
-            cis_lin(a, s, (float)(Dk * jout0 - plk) - kcf, er, ei);
-            cis_lin(0.f, s, (float)Dk, dr, di);
+            cis_lin(rc0, rsl, (float)(Dk * jout0 - plk) - kcf, er, ei);
+            cis_lin(0.f, rsl, (float)Dk, dr, di);
             // the row's pulse taps (uniform): output m - NWP + 1 + s takes tap Dp (NWP - 1 - s) + ph
             const float* hp = hp_t + pph * NWP;
             #pragma unroll
@@ -245,6 +291,9 @@ __global__ __launch_bounds__(64 * JT_MAX) void k_level_mfma(
             }
         }
         if (++pph == Dp) { end_period(); pph = 0; }
+        if (more) store_row(b ^ 1);
+        rc0 = nc0;
+        rsl = nsl;
         __syncthreads();
     }
     // rows past the last read: finish the open period, then every period until the segment's last output is stored
@@ -281,6 +330,9 @@ bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int
     PS = 16 * JT + NWp + 1;
     lds = (size_t)4 * 4 * NPH * PS + (size_t)2 * KS * 64 * 2 * 16 + (size_t)16 * KS * sizeof(TapK) + (size_t)8 * KS * 4;
     if (lds > 80 * 1024) return false;
+    // the per-thread staging registers hold a row's window words and tap entries
+    const int nthr = 64 * JT;
+    if (NPH * (16 * JT + NWp) > MAXI * nthr || 16 * KS * CB > MAXG * nthr) return false;
     // segments of output pulses: enough blocks to fill the device, a halo below a fifth of the rows
     const long blocks0 = (long)Np * ((C + CB - 1) / CB) * njr;
     nseg = 1;
@@ -290,14 +342,17 @@ bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int
     return true;
 }
 
-// whether a former takes this form for a level: where it fits and FSH_LEVEL_KERNEL=mfma asks for it
+// whether a former takes this form for a level: by default where it fits and one block covers a row's outputs (no
+// staging repeated across output ranges), the two-pass form being faster on longer rows; FSH_LEVEL_KERNEL=mfma
+// wherever it fits, =split or =fused never
 bool level_mfma_preferred(int C, int Lk, int Dk, int Ko, int Lp, int Dp, int Po)
 {
     const char* force = getenv("FSH_LEVEL_KERNEL");
-    if (!force || strcmp(force, "mfma") != 0) return false;
+    if (force && (strcmp(force, "split") == 0 || strcmp(force, "fused") == 0)) return false;
     int KS, NWp, NPH, PS, JT, njr, nseg, IS, nwp;
     size_t lds;
-    return level_mfma_shape(1, C, Lk, Dk, Ko, Lp, Dp, Po, KS, NWp, NPH, PS, JT, njr, nseg, IS, lds, nwp);
+    if (!level_mfma_shape(1, C, Lk, Dk, Ko, Lp, Dp, Po, KS, NWp, NPH, PS, JT, njr, nseg, IS, lds, nwp)) return false;
+    return (force && strcmp(force, "mfma") == 0) || njr == 1;
 }
 
 // hp_t: the pulse filter's polyphase table for level_mfma_nw(Lp, Dp) accumulators (launch_polyphase_nw)

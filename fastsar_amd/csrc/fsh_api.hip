@@ -109,7 +109,7 @@ struct Level {
     long long below = 1;           // tiles per parent... children per first-level child at this level's input
 };
 
-enum { ST_PREP, ST_ROT, ST_FIRP, ST_PHASE, ST_GEOM, ST_FINAL, ST_LEVEL, ST_N };
+enum { ST_PREP, ST_ROT, ST_FIRP, ST_PHASE, ST_GEOM, ST_FINAL, ST_LEVEL, ST_LMFMA, ST_N };
 
 struct Former {
     fsh_plan_desc p;
@@ -264,7 +264,7 @@ struct Former {
                 if ((st = launch_level_mfma(cur, (int)parents, h.P, h.K, c0, sl, ch, L.taps_k, h.Lk, h.Dk, h.plk, h.Ko, L.hp_t,
                                             h.Lp, h.Dp, h.plp, h.Po, h.kc, buf[yb], stream)))
                     return st;
-                mark_end(ST_LEVEL);
+                mark_end(ST_LMFMA);
                 cur = buf[yb];
                 curbuf = yb;
                 first_parent = (l == 0) ? g0 : first_parent * h.C;
@@ -410,10 +410,10 @@ int fsh_level_mfma(const float* x, int Np, int P, int K, const float* c0, const 
                    void* stream)
 {
     if (!x || !c0 || !sl || !z || !taps_k || !taps_p) { set_error("level_mfma: null argument"); return E_INVALID; }
-    int KS, NWp, NPH, PS, JT, njr, nseg, IS, nwp;
+    int KS, NWp, NPH, PS, JT, njr, nseg, IS, nwp, tpw, nwv;
     size_t lds;
     if (Np < 1 || Lk < 1 || Lp < 1 || !level_mfma_shape(Np, Cn, Lk, Dk, Ko, Lp, Dp, Po, KS, NWp, NPH, PS, JT, njr, nseg, IS,
-                                                        lds, nwp)) {
+                                                        lds, nwp, tpw, nwv)) {
         set_error("level_mfma: the level does not fit the fused matrix-core form (Dk, Dp >= 2, Ko >= 16, Dk ceil(Lk / Dk) "
                   "<= 128, ceil(Lp / Dp) <= 16)");
         return E_INVALID;

@@ -25,7 +25,7 @@ int launch_polyphase_nw(const float* h, int L, int D, int NW, float* t, hipStrea
 // filter's polyphase table for level_mfma_nw(Lp, Dp) accumulators
 int level_mfma_nw(int Lp, int Dp);
 bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int Po, int& KS, int& NWp, int& NPH, int& PS,
-                      int& JT, int& njr, int& nseg, int& IS, size_t& lds, int& nwp);
+                      int& JT, int& njr, int& nseg, int& IS, size_t& lds, int& nwp, int& tpw, int& nwv);
 bool level_mfma_preferred(int C, int Lk, int Dk, int Ko, int Lp, int Dp, int Po);
 int launch_level_mfma(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int C, const float* hk,
                       int Lk, int Dk, int plk, int Ko, const float* hp_t, int Lp, int Dp, int plp, int Po, double kc,

@@ -4,8 +4,9 @@
 //   y_c[p, j] = rho_c(p, j) sum_r x[p, Dk j + r - plk] g_c,p[r]     (g_c,p[r] = hk[r] exp(2 pi i sl_c[p] r))
 //   z_c[i, j] = sum_r hp[r] y_c[Dp i + r - plp, j]
 //
-// Block: one parent, a block of CB = 8 children, a range of JT output tiles of 16 (one wave each) and a segment of
-// output pulses [i0, i1); it walks the input rows Dp i0 - plp .. Dp (i1 - 1) - plp + Lp - 1 in order. Per row:
+// Block: one parent, a block of CB = 8 children, a range of JT output tiles of 16 (TPW per wave, one per wave when
+// there are at most 10) and a segment of output pulses [i0, i1); it walks the input rows
+// Dp i0 - plp .. Dp (i1 - 1) - plp + Lp - 1 in order. Per row:
 //   * the row's window is staged into LDS as polyphase planes of packed bfloat16 (Re, Im) words, hi and lo (the
 //     contraction runs over the taps in the order ph NWp + u for tap Dk u + ph, so a lane reads its contraction
 //     indices as adjacent word pairs and a half-wave reads consecutive words);
@@ -67,8 +68,8 @@ __device__ __forceinline__ void split2(float a, float b, unsigned int& hi, unsig
     lo = __builtin_bit_cast(unsigned int, l);
 }
 
-constexpr int MAXI = 4;                    // window words staged per thread and row (at most)
 constexpr int MAXG = 4;                    // tap entries built per thread and row (at most)
+constexpr int WAVES_MULTI = 8;             // waves of a block whose waves take several tiles
 
 struct TapK {                              // contraction index kappa: its tap value and tap number
     float h, r;
@@ -76,12 +77,14 @@ struct TapK {                              // contraction index kappa: its tap v
 
 }  // namespace
 
-template <int KS, int NWP>
-__global__ __launch_bounds__(64 * JT_MAX) void k_level_mfma(
+template <int KS, int NWP, int TPW>
+__global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI) void k_level_mfma(
     const float2* __restrict__ x, int P, int K, const float* __restrict__ c0, const float* __restrict__ sl, int C,
     const float* __restrict__ hk, int Lk, int Dk, int plk, int Ko, int NWp, int NPH, int PS, int JT, int njr,
     const float* __restrict__ hp_t, int Dp, int plp, int Lp, int Po, int IS, int nseg, float kcf, float2* __restrict__ z)
 {
+    constexpr int MAXI = TPW == 1 ? 4 : 8;             // window words staged per thread and row (at most)
+    const int NWV = blockDim.x >> 6;                   // waves: wave w takes tiles w, w + NWV, ... of the block's JT
     extern __shared__ unsigned int lds_u[];
     const int nthr = blockDim.x;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
@@ -193,18 +196,22 @@ __global__ __launch_bounds__(64 * JT_MAX) void k_level_mfma(
         off[s][0] = taba[8 * s + 2 * grp];
         off[s][1] = taba[8 * s + 2 * grp + 1];
     }
-    const int jt = jr * JT + wv;                         // global output tile of this wave
-    const int jl = 16 * wv + lc;                         // A row (output) of this lane within the block's window
-    const int jout0 = 16 * jt + 4 * grp;                 // the lane's four outputs (D rows), 0 .. 3 after it
+    // tile k of this wave: block tile wv + k NWV; its A rows start at word 16 (wv + k NWV) of the window, its D rows
+    // (the lane's four outputs) at jout0 + 16 k NWV
+    const int jout0 = 16 * (jr * JT + wv) + 4 * grp;
     const int c_l = lc >> 1, t = lc & 1, cg = cb * CB + c_l;
     const bool cok = cg < C;
-    const bool wact = jt * 16 < Ko;                      // a wave past the last tile only helps stage rows
-
-    float acc_p[NWP][4];
+    bool tact[TPW];                                      // tiles past the last output: the wave only stages rows
     #pragma unroll
-    for (int s = 0; s < NWP; ++s)
+    for (int k = 0; k < TPW; ++k) tact[k] = wv + k * NWV < JT && 16 * (jr * JT + wv + k * NWV) < Ko;
+
+    float acc_p[TPW][NWP][4];
+    #pragma unroll
+    for (int k = 0; k < TPW; ++k)
         #pragma unroll
-        for (int i = 0; i < 4; ++i) acc_p[s][i] = 0.f;
+        for (int s = 0; s < NWP; ++s)
+            #pragma unroll
+            for (int i = 0; i < 4; ++i) acc_p[k][s][i] = 0.f;
     // rows p = Dp m - plp + ph: period m, phase ph; slot s holds output m - NWP + 1 + s during period m
     const int pbeg = Dp * i0 - plp;
     const int pend = Dp * (i1 - 1) - plp + Lp;           // one past the last row any output of the segment reads
@@ -212,17 +219,21 @@ __global__ __launch_bounds__(64 * JT_MAX) void k_level_mfma(
     float* zc = reinterpret_cast<float*>(z + ((size_t)n * C + (cok ? cg : 0)) * Po * Ko) + t;
     auto end_period = [&]() {
         const int done = pm - NWP + 1;
-        if (wact && cok && done >= i0 && done < i1) {
+        #pragma unroll
+        for (int k = 0; k < TPW; ++k) {
+            if (tact[k] && cok && done >= i0 && done < i1) {
+                const int j0 = jout0 + 16 * k * NWV;
+                #pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (j0 + i < Ko) zc[2 * ((size_t)done * Ko + j0 + i)] = acc_p[k][0][i];
+            }
             #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                if (jout0 + i < Ko) zc[2 * ((size_t)done * Ko + jout0 + i)] = acc_p[0][i];
+            for (int s = 0; s < NWP - 1; ++s)
+                #pragma unroll
+                for (int i = 0; i < 4; ++i) acc_p[k][s][i] = acc_p[k][s + 1][i];
+            #pragma unroll
+            for (int i = 0; i < 4; ++i) acc_p[k][NWP - 1][i] = 0.f;
         }
-        #pragma unroll
-        for (int s = 0; s < NWP - 1; ++s)
-            #pragma unroll
-            for (int i = 0; i < 4; ++i) acc_p[s][i] = acc_p[s + 1][i];
-        #pragma unroll
-        for (int i = 0; i < 4; ++i) acc_p[NWP - 1][i] = 0.f;
         ++pm;
     };
 
@@ -251,7 +262,13 @@ __global__ __launch_bounds__(64 * JT_MAX) void k_level_mfma(
             load_row(p + 1);
             load_rot(p + 1, nc0, nsl);
         }
-        if (wact) {
+        const float* hp = hp_t + pph * NWP;              // the row's pulse taps (uniform)
+        float drr = 0.f, dri = 0.f;
+        if (tact[0]) cis_lin(0.f, rsl, (float)Dk, drr, dri);
+        #pragma unroll
+        for (int kt = 0; kt < TPW; ++kt) {
+          if (tact[kt]) {
+            const int jl = 16 * (wv + kt * NWV) + lc;      // A row (output) of this lane within the block's window
             const unsigned int* XH = planes + (2 * b) * NPH * PS;
             const unsigned int* XL = XH + NPH * PS;
             const uint4_t* F = frag + (size_t)b * KS * 64 * 2;
@@ -272,23 +289,22 @@ __global__ __launch_bounds__(64 * JT_MAX) void k_level_mfma(
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bl, acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, Bh, acc, 0, 0, 0);
             }
-            // rho_c(p, j) at the lane's outputs jout0 .. jout0 + 3: the lane pair (t = 0, 1) holds (Re, Im)
-            float er, ei, dr, di;
-            cis_lin(rc0, rsl, (float)(Dk * jout0 - plk) - kcf, er, ei);
-            cis_lin(0.f, rsl, (float)Dk, dr, di);
-            // the row's pulse taps (uniform): output m - NWP + 1 + s takes tap Dp (NWP - 1 - s) + ph
-            const float* hp = hp_t + pph * NWP;
+            // rho_c(p, j) at the lane's outputs j0 .. j0 + 3: the lane pair (t = 0, 1) holds (Re, Im); output
+            // m - NWP + 1 + s of the pulse filter takes tap Dp (NWP - 1 - s) + ph of this row
+            float er, ei;
+            cis_lin(rc0, rsl, (float)(Dk * (jout0 + 16 * kt * NWV) - plk) - kcf, er, ei);
             #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const float mine = acc[i];
                 const float other = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, mine), 0xB1, 0xF, 0xF, false));
                 const float v = fmaf(mine, er, other * (t ? ei : -ei));
                 #pragma unroll
-                for (int sl2 = 0; sl2 < NWP; ++sl2) acc_p[sl2][i] = fmaf(hp[sl2], v, acc_p[sl2][i]);
-                const float nr = fmaf(er, dr, -(ei * di));
-                ei = fmaf(er, di, ei * dr);
+                for (int sl2 = 0; sl2 < NWP; ++sl2) acc_p[kt][sl2][i] = fmaf(hp[sl2], v, acc_p[kt][sl2][i]);
+                const float nr = fmaf(er, drr, -(ei * dri));
+                ei = fmaf(er, dri, ei * drr);
                 er = nr;
             }
+          }
         }
         if (++pph == Dp) { end_period(); pph = 0; }
         if (more) store_row(b ^ 1);
@@ -314,7 +330,7 @@ int level_mfma_nw(int Lp, int Dp)
 }
 
 bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int Po, int& KS, int& NWp, int& NPH, int& PS,
-                      int& JT, int& njr, int& nseg, int& IS, size_t& lds, int& nwp)
+                      int& JT, int& njr, int& nseg, int& IS, size_t& lds, int& nwp, int& tpw, int& nwv)
 {
     if (Dk < 2 || Dp < 2 || C < 1 || Ko < 16) return false;
     const int NW = (Lk + Dk - 1) / Dk;
@@ -324,15 +340,25 @@ bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int
     NPH = (16 * KS + NWp - 1) / NWp;
     nwp = level_mfma_nw(Lp, Dp);
     if (nwp < 0) return false;
+    // output tiles: one per wave up to JT_MAX, else WAVES_MULTI waves with up to three each per block
     const int MT = (Ko + 15) / 16;
-    njr = (MT + JT_MAX - 1) / JT_MAX;
-    JT = (MT + njr - 1) / njr;
+    if (MT <= JT_MAX) {
+        njr = 1;
+        JT = MT;
+        tpw = 1;
+        nwv = JT;
+    } else {
+        njr = (MT + 3 * WAVES_MULTI - 1) / (3 * WAVES_MULTI);
+        JT = (MT + njr - 1) / njr;
+        nwv = WAVES_MULTI;
+        tpw = (JT + nwv - 1) / nwv;
+    }
     PS = 16 * JT + NWp + 1;
     lds = (size_t)4 * 4 * NPH * PS + (size_t)2 * KS * 64 * 2 * 16 + (size_t)16 * KS * sizeof(TapK) + (size_t)8 * KS * 4;
     if (lds > 80 * 1024) return false;
     // the per-thread staging registers hold a row's window words and tap entries
-    const int nthr = 64 * JT;
-    if (NPH * (16 * JT + NWp) > MAXI * nthr || 16 * KS * CB > MAXG * nthr) return false;
+    const int nthr = 64 * nwv, maxi = tpw == 1 ? 4 : 8;
+    if (NPH * (16 * JT + NWp) > maxi * nthr || 16 * KS * CB > MAXG * nthr) return false;
     // segments of output pulses: enough blocks to fill the device, a halo below a fifth of the rows
     const long blocks0 = (long)Np * ((C + CB - 1) / CB) * njr;
     nseg = 1;
@@ -342,17 +368,18 @@ bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int
     return true;
 }
 
-// whether a former takes this form for a level: by default where it fits and one block covers a row's outputs (no
-// staging repeated across output ranges), the two-pass form being faster on longer rows; FSH_LEVEL_KERNEL=mfma
+// whether a former takes this form for a level: by default where it fits with one block of one tile per wave
+// covering a row's outputs (no staging repeated across output ranges; the blocks of several tiles per wave run too
+// few waves per CU to hide the per-row barrier), the two-pass form being faster on longer rows; FSH_LEVEL_KERNEL=mfma
 // wherever it fits, =split or =fused never
 bool level_mfma_preferred(int C, int Lk, int Dk, int Ko, int Lp, int Dp, int Po)
 {
     const char* force = getenv("FSH_LEVEL_KERNEL");
     if (force && (strcmp(force, "split") == 0 || strcmp(force, "fused") == 0)) return false;
-    int KS, NWp, NPH, PS, JT, njr, nseg, IS, nwp;
+    int KS, NWp, NPH, PS, JT, njr, nseg, IS, nwp, tpw, nwv;
     size_t lds;
-    if (!level_mfma_shape(1, C, Lk, Dk, Ko, Lp, Dp, Po, KS, NWp, NPH, PS, JT, njr, nseg, IS, lds, nwp)) return false;
-    return (force && strcmp(force, "mfma") == 0) || njr == 1;
+    if (!level_mfma_shape(1, C, Lk, Dk, Ko, Lp, Dp, Po, KS, NWp, NPH, PS, JT, njr, nseg, IS, lds, nwp, tpw, nwv)) return false;
+    return (force && strcmp(force, "mfma") == 0) || (njr == 1 && tpw == 1);
 }
 
 // hp_t: the pulse filter's polyphase table for level_mfma_nw(Lp, Dp) accumulators (launch_polyphase_nw)
@@ -360,29 +387,34 @@ int launch_level_mfma(const float2* x, int Np, int P, int K, const float* c0, co
                       int Lk, int Dk, int plk, int Ko, const float* hp_t, int Lp, int Dp, int plp, int Po, double kc,
                       float2* z, hipStream_t s)
 {
-    int KS, NWp, NPH, PS, JT, njr, nseg, IS, nwp;
+    int KS, NWp, NPH, PS, JT, njr, nseg, IS, nwp, tpw, nwv;
     size_t lds;
     if (Np < 1 || P < 1 || K < 1 || plk < 0 || plp < 0 ||
-        !level_mfma_shape(Np, C, Lk, Dk, Ko, Lp, Dp, Po, KS, NWp, NPH, PS, JT, njr, nseg, IS, lds, nwp)) {
+        !level_mfma_shape(Np, C, Lk, Dk, Ko, Lp, Dp, Po, KS, NWp, NPH, PS, JT, njr, nseg, IS, lds, nwp, tpw, nwv)) {
         set_error("level_mfma: the level does not fit the fused matrix-core form");
         return E_INVALID;
     }
     const long G = (long)Np * nseg * ((C + CB - 1) / CB) * njr;
     if (G > 2147483647L) { set_error("level_mfma: too many blocks"); return E_INVALID; }
-    const dim3 grid((unsigned)G), block((unsigned)(64 * JT));
+    const dim3 grid((unsigned)G), block((unsigned)(64 * nwv));
     const float kcf = (float)kc;
-#define FSH_LM(A, B) hipLaunchKernelGGL((k_level_mfma<A, B>), grid, block, lds, s, x, P, K, c0, sl, C, hk, Lk, Dk, plk, Ko, \
-                                        NWp, NPH, PS, JT, njr, hp_t, Dp, plp, Lp, Po, IS, nseg, kcf, z)
+#define FSH_LM(A, B, T) hipLaunchKernelGGL((k_level_mfma<A, B, T>), grid, block, lds, s, x, P, K, c0, sl, C, hk, Lk, Dk, plk, \
+                                           Ko, NWp, NPH, PS, JT, njr, hp_t, Dp, plp, Lp, Po, IS, nseg, kcf, z)
+#define FSH_LM_T(A, B)                                                                                                \
+    switch (tpw) {                                                                                                    \
+        case 1: FSH_LM(A, B, 1); break; case 2: FSH_LM(A, B, 2); break; default: FSH_LM(A, B, 3); break;             \
+    }
 #define FSH_LM_P(A)                                                                                                   \
     switch (nwp) {                                                                                                    \
-        case 4: FSH_LM(A, 4); break; case 6: FSH_LM(A, 6); break; case 8: FSH_LM(A, 8); break;                        \
-        case 10: FSH_LM(A, 10); break; case 12: FSH_LM(A, 12); break; default: FSH_LM(A, 16); break;                  \
+        case 4: FSH_LM_T(A, 4); break; case 6: FSH_LM_T(A, 6); break; case 8: FSH_LM_T(A, 8); break;                  \
+        case 10: FSH_LM_T(A, 10); break; case 12: FSH_LM_T(A, 12); break; default: FSH_LM_T(A, 16); break;            \
     }
     switch (KS) {
         case 1: FSH_LM_P(1); break; case 2: FSH_LM_P(2); break; case 3: FSH_LM_P(3); break; case 4: FSH_LM_P(4); break;
         case 5: FSH_LM_P(5); break; case 6: FSH_LM_P(6); break; case 7: FSH_LM_P(7); break; default: FSH_LM_P(8); break;
     }
 #undef FSH_LM_P
+#undef FSH_LM_T
 #undef FSH_LM
     FSH_TRY(hipGetLastError());
     return OK;

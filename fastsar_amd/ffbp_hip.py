@@ -14,7 +14,7 @@ from . import _hip
 from . import memory as _mem
 from .plan import C, HOST_LEVELS, level_phase_constants, tile_digits
 
-STAGES = ('prepare', 'rot_fir_k', 'fir_p', 'device_phases', 'final_geom', 'final_tiles', 'level_fused')
+STAGES = ('prepare', 'rot_fir_k', 'fir_p', 'device_phases', 'final_geom', 'final_tiles', 'level_fused', 'level_mfma')
 
 
 def _f32(a):

@@ -28,7 +28,8 @@ def stage_flops(plan):
                                                            times a complex sample per output and tap)
       pulse FIR             4 N C Po Ko Lp
       final stage           8 T^2 Pf Qf per tile (a real 2T x PfQf x 2T product)
-    and ('level_fused', l), the level's rotation and both filters in one pass, their sum."""
+    and ('level_fused', l) / ('level_mfma', l), the level's rotation and both filters in one pass (on VALU / with the
+    range filter on the matrix cores), their sum."""
     out = {}
     for l, (N, lv) in enumerate(level_counts(plan)):
         rot = 6.0 * N * lv['C'] * lv['P'] * lv['K']
@@ -37,6 +38,7 @@ def stage_flops(plan):
         if lv['Dp'] > 1:
             out[('fir_p', l)] = 4.0 * N * lv['C'] * lv['Po'] * lv['Ko'] * lv['fir_p']['L']
         out[('level_fused', l)] = out[('rot_fir_k', l)] + out.get(('fir_p', l), 0.0)
+        out[('level_mfma', l)] = out[('level_fused', l)]
     f = plan['final']
     ntiles = f['cen'].shape[0]
     out[('final_tiles', len(plan['levels']))] = 8.0 * plan['T'] ** 2 * f['P'] * f['K'] * ntiles
@@ -65,6 +67,7 @@ def stage_bytes(plan):
         if lv['Dp'] > 1:
             out[('fir_p', l)] = 8.0 * N * lv['C'] * (lv['P'] + lv['Po']) * lv['Ko']
         out[('level_fused', l)] = 8.0 * N * (lv['P'] * lv['K'] + lv['C'] * lv['P'] + lv['C'] * lv['Po'] * lv['Ko'])
+        out[('level_mfma', l)] = out[('level_fused', l)]
     f = plan['final']
     ntiles = f['cen'].shape[0]
     out[('final_tiles', len(plan['levels']))] = (8.0 * f['P'] * f['K'] + 16.0 * f['P']) * ntiles + 8.0 * plan['nx'] * plan['ny']

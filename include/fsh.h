@@ -145,7 +145,7 @@ int fsh_ffbp_set_group(void* handle, int ng);
 /* The group used by the last call and the device bytes the former holds. */
 int fsh_ffbp_info(void* handle, int* ng_used, size_t* device_bytes);
 /* Per-stage device times (ms) of the last call: prepare, rot_fir_k, fir_p, device_phases, final_geom, final_tiles,
- * level_fused. */
+ * level_fused, level_mfma. */
 int fsh_ffbp_stage_ms(void* handle, float* ms, int n);
 /* Profiling (no reference counterpart; the analog of ffbp_cuda.PROFILE, fastsar/ffbp_cuda.py:635-652): when on,
  * every launch is bracketed by events on the former's stream and waited for; fsh_ffbp_profile then gives, per

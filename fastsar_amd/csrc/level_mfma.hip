@@ -111,13 +111,19 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI) void k_l
     TapK* tabk = reinterpret_cast<TapK*>(frag + 2 * KS * 64 * 2);
     int* taba = reinterpret_cast<int*>(tabk + 16 * KS);  // [8 KS] pair offsets ph PS + u
 
+    // contraction slot sigma = 16 s + 4 g + ew (k-step s, lane group g, word ew = 2 pr + e of the lane's operand)
+    // holds tap (ph, u) = (4 pb + g, 2 up + e), where pr-pair column 2 s + pr = pb (NWp / 2) + up: the four lane groups
+    // of a k-step read four consecutive planes at the same word, and planes are PS = 16 mod 64 words apart, so their
+    // 16-word runs fall on disjoint banks
     for (int i = tid; i < 16 * KS; i += nthr) {
-        const int ph = i / NWp, u = i - ph * NWp, r = Dk * u + ph;
-        TapK e;
-        e.h = (ph < Dk && r < Lk) ? hk[r] : 0.f;
-        e.r = (float)r;
-        tabk[i] = e;
-        if ((i & 1) == 0) taba[i >> 1] = ph * PS + u;
+        const int st = i >> 4, g = (i >> 2) & 3, pr = (i >> 1) & 1, e = i & 1;
+        const int col = 2 * st + pr, pb = col / (NWp >> 1), up = col - pb * (NWp >> 1);
+        const int ph = 4 * pb + g, u = 2 * up + e, r = Dk * u + ph;
+        TapK t;
+        t.h = (ph < Dk && r < Lk) ? hk[r] : 0.f;
+        t.r = (float)r;
+        tabk[i] = t;
+        if (e == 0) taba[i >> 1] = ph * PS + u;
     }
     __syncthreads();
 
@@ -133,9 +139,13 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI) void k_l
         const int idx = tid + k * nthr;
         const int ph = idx / WM, m = idx - ph * WM;
         const int kk = Dk * (jbase + m) + ph - plk;
-        const bool ok = idx < NPH * WM;
+        const bool ok = idx < Dk * WM;                 // the planes past Dk hold zeros, written once below
         it_lds[k] = ok ? ph * PS + m : -1;
-        it_k[k] = (ok && ph < Dk && kk >= 0 && kk < K) ? kk : -1;
+        it_k[k] = (ok && kk >= 0 && kk < K) ? kk : -1;
+    }
+    for (int i = tid; i < (NPH - Dk) * PS; i += nthr) {
+        #pragma unroll
+        for (int hb = 0; hb < 4; ++hb) planes[(size_t)hb * NPH * PS + Dk * PS + i] = 0u;
     }
     const int gc = tid & 7, gcg = cb * CB + gc;
     float g_h[MAXG], g_r[MAXG];
@@ -335,9 +345,11 @@ bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int
     if (Dk < 2 || Dp < 2 || C < 1 || Ko < 16) return false;
     const int NW = (Lk + Dk - 1) / Dk;
     NWp = NW + (NW & 1);
-    KS = (Dk * NWp + 15) / 16;
+    // phases padded to a multiple of 4 (one per lane group), NWp / 2 pairs each, two pair columns per k-step
+    const int cols = ((Dk + 3) / 4) * (NWp / 2);
+    KS = (cols + 1) / 2;
     if (KS < 1 || KS > 8) return false;
-    NPH = (16 * KS + NWp - 1) / NWp;
+    NPH = 4 * ((2 * KS + NWp / 2 - 1) / (NWp / 2));
     nwp = level_mfma_nw(Lp, Dp);
     if (nwp < 0) return false;
     // output tiles: one per wave up to JT_MAX, else WAVES_MULTI waves with up to three each per block
@@ -353,12 +365,13 @@ bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int
         nwv = WAVES_MULTI;
         tpw = (JT + nwv - 1) / nwv;
     }
-    PS = 16 * JT + NWp + 1;
+    PS = 16 * JT + NWp;
+    PS = ((PS - 16 + 63) / 64) * 64 + 16;                  // 16 mod 64 words: the lane groups' runs on disjoint banks
     lds = (size_t)4 * 4 * NPH * PS + (size_t)2 * KS * 64 * 2 * 16 + (size_t)16 * KS * sizeof(TapK) + (size_t)8 * KS * 4;
     if (lds > 80 * 1024) return false;
     // the per-thread staging registers hold a row's window words and tap entries
     const int nthr = 64 * nwv, maxi = tpw == 1 ? 4 : 8;
-    if (NPH * (16 * JT + NWp) > maxi * nthr || 16 * KS * CB > MAXG * nthr) return false;
+    if (Dk * (16 * JT + NWp) > maxi * nthr || 16 * KS * CB > MAXG * nthr) return false;
     // segments of output pulses: enough blocks to fill the device, a halo below a fifth of the rows
     const long blocks0 = (long)Np * ((C + CB - 1) / CB) * njr;
     nseg = 1;

@@ -104,7 +104,7 @@ def test_level_fused_stage(Dk, pk, Dp, pp, Cn, Np, P, K):
     assert e < -115, e
 
 
-MFMA_LEVEL_CASES = [(9, 0.6, 11, 0.6, 64, 1, 300, 600), (8, 0.6, 8, 0.6, 64, 2, 200, 315), (3, 0.5, 4, 0.7, 13, 2, 150, 200),
+MFMA_LEVEL_CASES = [(9, 0.5, 11, 0.6, 64, 1, 300, 600), (8, 0.6, 8, 0.6, 64, 2, 200, 315), (3, 0.5, 4, 0.7, 13, 2, 150, 200),
                     (4, 0.6, 4, 0.6, 8, 1, 2000, 100), (2, 0.6, 2, 0.6, 8, 1, 100, 700), (3, 0.4, 6, 0.6, 24, 3, 260, 90)]
 
 

@@ -19,6 +19,7 @@ import warnings
 
 import numpy as np
 
+from . import gate as _gate
 from . import memory as _mem
 from . import plan as _plan
 from .sim import Collect
@@ -153,7 +154,9 @@ class ImageFormer:
     T: 'auto' (the largest of 32 and 16 whose predicted error meets target_db) or 4, 8, 16, 32. levels: None
     (three, more where three cannot split an axis) or a count. aperture_weight(points, pulses) -> [n, m]: per-pixel
     pulse weights applied in the final stage. ref [P]: each pulse's reference range when it is not |ant|.
-    precision: 'float32'. range_gate: 'auto' or False (the hip first level holds any filter window).
+    precision: 'float32'. range_gate: True cuts each pulse to the range bins the image spans before formation (a
+    wide-band history for a narrow image: fewer samples, a first level that decimates less); 'auto' and False do
+    not gate, since the hip first level holds any range filter.
     atten: stopband attenuation (dB) of the levels' Kaiser decimation filters; 70 (the reference's design, whose
     plans this reproduces bit for bit) sets an image error floor near -60 dB, 80 lowers it by about 10 dB for
     filters about 16 percent longer."""
@@ -197,23 +200,33 @@ class ImageFormer:
             raise ValueError('ref has non-finite values')
         if range_gate not in ('auto', True, False):
             raise ValueError(f"range_gate must be 'auto', True or False, got {range_gate!r}")
-        if range_gate is True:
-            raise ValueError('range_gate=True is not available on the hip backend yet (its first level holds any '
-                             "filter window, so 'auto' never gates)")
         if padded is not None and (aperture_weight is None or levels is None or len(padded[2]) != levels
                                    or not 0 <= padded[0] <= self.P - padded[1]):
             raise ValueError('padded=(first, count, pulse) goes with an aperture_weight, the levels of the plan that '
                              f'pulse is from and an aperture within the {self.P} pulses, got {padded!r}')
-        while True:
-            try:
-                plan = _plan.make_plan(col, nx, ny, spx, spy, T=T, nlev=nlev, pmax=pmax, atten=float(atten), e1=e1, e2=e2,
-                                       pulse=padded and padded[2])
-                break
-            except ValueError as e:
-                if levels is not None or 'cannot split' not in str(e) or nlev >= 8:
-                    raise
-                nlev += 1
+        def planned(col, nlev):
+            while True:
+                try:
+                    return _plan.make_plan(col, nx, ny, spx, spy, T=T, nlev=nlev, pmax=pmax, atten=float(atten), e1=e1,
+                                           e2=e2, pulse=padded and padded[2]), nlev
+                except ValueError as e:
+                    if levels is not None or 'cannot split' not in str(e) or nlev >= 8:
+                        raise
+                    nlev += 1
+        plan, nlev = planned(col, nlev)
         self.range_gate = None
+        self._gate = self._gated = None
+        if range_gate is True:
+            # 'auto' gates only where a backend's first level cannot hold the range filter, which the hip first
+            # level always can; True gates wherever the image spans less than the swath (fastsar/api.py:404-417)
+            g = _gate.gate_plan(self.ant, ref, float(fmin), float(df), self.K, nx, ny, spx, spy, e1, e2)
+            kept = g['k1'] - g['k0']
+            if g['N'] < g['Kp'] and kept < self.K // 2:
+                gcol = Collect(fmin=g['fmin'], df=g['df'], K=kept, ant=self.ant, res=0.5)
+                plan, nlev = planned(gcol, 3 if levels is None else int(levels))
+                self.range_gate = dict(g, K=self.K, kept=kept, note=(
+                    f'range gate: each pulse cut to the {g["N"]} range bins about the image before formation, '
+                    f'{kept} samples for {self.K}'))
         self._Kf = plan['levels'][0]['K']
         self.levels = nlev
         self._plan = plan
@@ -222,8 +235,14 @@ class ImageFormer:
         wf = None if aperture_weight is None else final_weights(plan, aperture_weight, grad=True, **wspan)
         if window:
             self.wp, self.wk = _window(self.P, self.K)
+        win = (self.wp, self.wk) if window else None
         from .ffbp_hip import make_ffbp_hip
-        self._form = make_ffbp_hip(plan, coll, wf=wf, window=(self.wp, self.wk) if window else None)
+        if self.range_gate is not None:
+            # the gate windows the history; the former takes the gated one as it is
+            self._gate = _gate.RangeGate(self.range_gate, self.P, self.K, window=win)
+            self._form = make_ffbp_hip(plan, coll, wf=wf, window=None)
+        else:
+            self._form = make_ffbp_hip(plan, coll, wf=wf, window=win)
 
     def stage(self, S):
         """The hip former takes the history as it is: former(former.stage(S)) equals former(S). (fastsar/api.py:471-480)"""
@@ -233,13 +252,18 @@ class ImageFormer:
         """S [P, K] complex phase history (numpy, or a fastsar_amd._hip.DeviceArray) -> complex64 image [nx, ny].
         The window and the scan for NaN and inf run on the device. out: an optional DeviceArray for the image."""
         from . import _hip
-        if isinstance(S, _hip.DeviceArray):
-            return self._form(S, out=out)
-        S = np.asarray(S)
-        if S.shape != (self.P, self.K):
-            raise ValueError(f'phase history must be {(self.P, self.K)}, got {S.shape}')
-        _check_history(S, finite=False)
-        return self._form(np.ascontiguousarray(S, np.complex64), out=out)
+        if not isinstance(S, _hip.DeviceArray):
+            S = np.asarray(S)
+            if S.shape != (self.P, self.K):
+                raise ValueError(f'phase history must be {(self.P, self.K)}, got {S.shape}')
+            _check_history(S, finite=False)
+            S = np.ascontiguousarray(S, np.complex64)
+        if self._gate is not None:
+            # gated on the device (windowed and scanned for NaN and inf there) into a buffer the former keeps
+            if self._gated is None:
+                self._gated = _hip.DeviceArray((self.P, self.range_gate['kept']), np.complex64)
+            S = self._gate(S, out=self._gated)
+        return self._form(S, out=out)
 
     def memory(self):
         """dict(backend, needed, available, full_speed, parts) in bytes. (fastsar/api.py:643-649)"""
@@ -247,6 +271,10 @@ class ImageFormer:
         info = self._form.info()
         dev = _hip.device_info()
         parts = dict(history=8 * self.P * self._Kf, formation=info['device_bytes'])
+        if self.range_gate is not None:
+            g = self.range_gate
+            chunk = max(1, min(self.P, (1 << 30) // (8 * (g['Kp'] + g['N'] + self.K))))
+            parts['range_gate'] = 8 * chunk * (g['Kp'] + g['N'] + self.K)
         need = float(sum(parts.values()))
         return dict(backend=self.backend, needed=need, available=float(dev['mem']), full_speed=need <= dev['mem'], parts=parts)
 

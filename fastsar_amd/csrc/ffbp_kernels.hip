@@ -109,6 +109,53 @@ __global__ __launch_bounds__(256) void k_rot_fir_k(const float2* __restrict__ x,
     }
 }
 
+// The same stage for a range filter whose window no tiling of k_rot_fir_k holds in LDS (a large range decimation:
+// an image narrow in range from a wide-band history). One wave forms one output (row p, sample j, child c) at a time:
+// its 64 lanes take taps r = lane + 64 i, reading the row directly (64 consecutive samples per step, coalesced),
+// and the sum is reduced across the wave. The ramp of tap r is seeded in float64 at r = lane and advanced by a
+// float32 complex recurrence 64 taps apart (at most L / 64 steps). grid (ceil(Ko / JW), min(P, 65535), Np), block
+// 256: the four waves of a block share JW outputs of a row (rows p, p + gridDim.y, ...).
+__global__ __launch_bounds__(256) void k_rot_fir_k_long(const float2* __restrict__ x, int P, int K, const float* __restrict__ c0,
+                                                        const float* __restrict__ sl, int Cn, const float* __restrict__ taps,
+                                                        int L, int pl, int D, int Ko, double kc, int JW, float2* __restrict__ y)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int n = blockIdx.z;
+    const int j1 = min(Ko, (int)(blockIdx.x + 1) * JW);
+    for (int p = blockIdx.y; p < P; p += gridDim.y)
+    for (int c = 0; c < Cn; ++c) {
+        const float2* xr = x + ((size_t)n * P + p) * K;
+        const double a = (double)c0[((size_t)n * Cn + c) * P + p];
+        const double b = (double)sl[((size_t)n * Cn + c) * P + p];
+        float sr, si;
+        cis_cyc(64.0 * b, sr, si);
+        for (int j = blockIdx.x * JW + wv; j < j1; j += 4) {
+            const int kj = D * j - pl;
+            float wr, wi;
+            cis_cyc(fma((double)(kj + lane) - kc, b, a), wr, wi);
+            float ar = 0.f, ai = 0.f;
+            for (int r = lane; r < L; r += 64) {
+                const int k = kj + r;
+                if (k >= 0 && k < K) {
+                    const float2 v = xr[k];
+                    const float t = taps[r];
+                    const float zr = fmaf(v.x, wr, -(v.y * wi)), zi = fmaf(v.x, wi, v.y * wr);
+                    ar = fmaf(t, zr, ar);
+                    ai = fmaf(t, zi, ai);
+                }
+                const float nr = fmaf(wr, sr, -(wi * si));
+                wi = fmaf(wr, si, wi * sr);
+                wr = nr;
+            }
+            for (int m = 32; m >= 1; m >>= 1) {
+                ar += __shfl_xor(ar, m, 64);
+                ai += __shfl_xor(ai, m, 64);
+            }
+            if (lane == 0) y[(((size_t)n * Cn + c) * P + p) * Ko + j] = make_float2(ar, ai);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------------ rot_fir_k on MFMA
 //
 // The same stage as a matrix product per parent row p. With k_j = D j - pl and the child's ramp factored at k_j,
@@ -783,7 +830,8 @@ int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, con
     // FSH_ROT_KERNEL=valu|mfma|mfma1 forces one (mfma: the row kernel where its taps fit; mfma1: the per-wave-row
     // form); by default the row kernel for rows of four or more M-tiles, the per-wave-row form below that
     const char* force = getenv("FSH_ROT_KERNEL");
-    const bool want_valu = force && strcmp(force, "valu") == 0;
+    const bool want_long = force && strcmp(force, "long") == 0;
+    const bool want_valu = (force && strcmp(force, "valu") == 0) || want_long;
     const bool want_mfma1 = force && strcmp(force, "mfma1") == 0;
     const bool want_mfma = (force && strcmp(force, "mfma") == 0) || want_mfma1;
     if (!want_valu && (D > 1 || want_mfma) && L <= 128) {
@@ -799,9 +847,17 @@ int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, con
     int JB = 64;
     auto need = [&](int jb) { const int W = D * (jb - 1) + L; return 2 * PB * W * 4 + 2 * PB * (W | 1) * 4 + L * 4; };
     while (JB > 4 && need(JB) > budget) JB /= 2;
-    if (need(JB) > budget) {
-        set_error("rot_fir_k: the filter window does not fit in LDS");
-        return E_INVALID;
+    if (need(JB) > budget || want_long) {
+        // a window longer than any LDS tile: the direct form (FSH_ROT_KERNEL=long forces it, for tests)
+        const int JW = 16;
+        for (int n0 = 0; n0 < Np; n0 += 65535) {
+            const int nn = std::min(65535, Np - n0);
+            dim3 grid((Ko + JW - 1) / JW, std::min(P, 65535), nn);
+            hipLaunchKernelGGL(k_rot_fir_k_long, grid, dim3(256), 0, s, x + (size_t)n0 * P * K, P, K, c0 + (size_t)n0 * Cn * P,
+                               sl + (size_t)n0 * Cn * P, Cn, taps, L, pl, D, Ko, kc, JW, y + (size_t)n0 * Cn * P * Ko);
+            FSH_TRY(hipGetLastError());
+        }
+        return OK;
     }
     const long gy = (P + PB - 1) / PB;
     if (gy > 2147483647L) { set_error("rot_fir_k: too many pulses"); return E_INVALID; }

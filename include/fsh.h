@@ -174,6 +174,25 @@ int fsh_exact_create(const fsh_exact_desc* desc, void** handle);
 int fsh_exact_form(void* handle, const void* S, int S_on_device, void* image, int image_on_device, int* bad_pulse);
 int fsh_exact_destroy(void* handle);
 
+/* ------------------------------------------------------------------------------------------ range gate
+ * A wide-band phase history cut to the range swath of an image (fastsar/api.py:272-300 plans it on the host;
+ * ImageFormer._gate, fastsar/api.py:525-564, and the cuda backend's range_gate apply it): each pulse windowed by
+ * wp, wk (or not, when NULL), padded by npl samples below to Kp, inverse-transformed, cut to its first N - N/2 and
+ * last N/2 range bins, transformed back (length N) and kept over [k0, k1) times taper / Kp (hipFFT does not
+ * scale; taper holds the plan's Kp / N). Host arrays of the descriptor are copied at create. */
+typedef struct fsh_gate_desc {
+    int P, K, npl, Kp, N, k0, k1;
+    const float* taper;            /* [k1 - k0] */
+    const float* wp;               /* [P] or NULL */
+    const float* wk;               /* [K] or NULL */
+} fsh_gate_desc;
+
+int fsh_gate_create(const fsh_gate_desc* desc, void** handle);
+/* S [P, K] complex64 (host or device) -> out [P, k1 - k0] complex64 (device); FSH_ENONFINITE with *bad_pulse, the
+ * first pulse holding a NaN or inf. */
+int fsh_gate_run(void* handle, const void* S, int S_on_device, float* out, int* bad_pulse);
+int fsh_gate_destroy(void* handle);
+
 /* Exact backprojection onto points in float64 (device pointers): S [P, K] complex64 windowed by wp, wk (or NULL),
  * profiles of nfft bins (a power of two) in complex128, exact float64 range and phase per point and pulse, linear
  * (cubic = 0) or cubic Lagrange reads; out [N] complex128 accumulates. The float64 path of

@@ -6,6 +6,7 @@ Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this
       rot_fir_k, fir_p, final_tiles  (fastsar/ffbp_cpu.cpp:147-469)
       bp_points  FP64 exact backprojection from float64 profiles (fastsar/bp.py:55-63, 99-138; tests/test_bp.py:23-35)
       exact_tiles  ExactFormer's tiled algorithm in float64 (fastsar/exact.py:240-328)
+      range_gate   the range gate of a wide-band history in complex128 (fastsar/api.py:546-563)
   * the reference's own CPU kernels, compiled in place from its sources into oracle/_ref/ (ref_* below);
   * the orchestration of the CPU former restated in Python (fastsar/ffbp_cpu.py:158-291), which drives either
     set of stages: ffbp_image(..., stages='ref') is the reference's CPU path, stages='oracle' the float64 one;
@@ -22,6 +23,9 @@ How each function is pinned (tests/test_oracle.py):
   ffbp_image                      stages='ref' against stages='oracle' (-100 dB), and against bp_points within the
                                   final-stage model's predicted error (tests/test_accuracy.py:29-54)
   window                          scipy.signal.windows.taylor, the library routine the reference calls
+  range_gate                      the image it forms: float64 exact backprojection of the gated and the whole history
+                                  agree to -98 dB (no fitted gain), and planted mistakes in the range cut, the
+                                  transform directions or the Kp / N scale each fail that check
 """
 import ctypes
 import os
@@ -174,6 +178,22 @@ def direct_sum(S, ant, fmin, df, pts, ref=None):
         dR = np.linalg.norm(flat - ant[p], axis=1) - ref[p]
         out += np.exp(4j * np.pi / C * dR[:, None] * f[None, :]) @ S[p]
     return out.reshape(np.shape(pts)[:-1])
+
+
+def range_gate(S, g, win=None):
+    """S [P, K] gated in range by the plan g (npl, Kp, N, k0, k1, taper) in complex128: windowed by win = (wp, wk)
+    when given, padded by npl samples below to Kp, inverse-transformed (numpy's 1 / Kp scale), cut to the first
+    N - N/2 and last N/2 range bins, transformed back and kept over [k0, k1) times the taper. -> [P, k1 - k0]
+    complex128. (fastsar/api.py:546-563)"""
+    S = np.asarray(S, np.complex128)
+    P, K = S.shape
+    npl, Kp, N, k0, k1 = g['npl'], g['Kp'], g['N'], g['k0'], g['k1']
+    h = N // 2
+    z = np.zeros((P, Kp), np.complex128)
+    z[:, npl:npl + K] = S if win is None else S * (np.asarray(win[0], np.float64)[:, None] * np.asarray(win[1], np.float64)[None, :])
+    q = np.fft.ifft(z, axis=1)
+    u = np.fft.fft(np.concatenate([q[:, :N - h], q[:, Kp - h:]], axis=1), axis=1)
+    return u[:, k0:k1] * np.asarray(g['taper'], np.float64)
 
 
 def plane_points(nx, ny, spx, spy, e1=(1.0, 0.0, 0.0), e2=(0.0, 1.0, 0.0), center=None):

@@ -293,3 +293,95 @@ def test_window_is_scipy_taylor():
     wp, wk = oracle.window(100, 64)
     assert np.array_equal(wp, taylor(100, nbar=4, sll=35, norm=False).astype(np.float32))
     assert np.array_equal(wk, taylor(64, nbar=4, sll=35, norm=False).astype(np.float32))
+
+
+# ------------------------------------------------------------------------------------------------ range gate
+
+def wide_band(K=6000, P=96, ncl=60, seed=3):
+    """An image narrow in range from a wide-band collection: 0.5 m resolution at 30 km, 40 degrees grazing, points
+    in a 19 m image and clutter over 2 km of range around it (the reference's wide_band fixture,
+    tests/test_api.py:132-148, at a quarter of its samples and pulses)."""
+    rng = np.random.default_rng(seed)
+    res, r0 = 0.5, 30e3
+    df = C / (2 * res) / K
+    y = (np.arange(P) - (P - 1) / 2) * (r0 * C / 9.6e9 / (2 * res) / P)
+    psi = np.radians(40.0)
+    ant = np.stack([np.full(P, -r0 * np.cos(psi)), y, np.full(P, r0 * np.sin(psi))], 1)
+    col = sim.Collect(fmin=9.6e9 - (K // 2) * df, df=df, K=K, ant=ant, res=res)
+    pts = np.array([[0.0, 0.0, 0], [5.3, -3.1, 0], [-7.2, 6.4, 0]])
+    cl = np.stack([(rng.random(ncl) - 0.5) * 2000, (rng.random(ncl) - 0.5) * 30, np.zeros(ncl)], 1)
+    amp = np.concatenate([np.full(3, 30.0), (rng.standard_normal(ncl) + 1j * rng.standard_normal(ncl)) / np.sqrt(2)])
+    S = sim.simulate_brute(col, np.concatenate([pts, cl]), amp).astype(np.complex64)
+    grid = dict(nx=48, ny=48, spx=0.4, spy=0.4, e1=(0.0, 1.0, 0.0), e2=(1.0, 0.0, 0.0))
+    return col, S, grid
+
+
+@pytest.fixture(scope='module')
+def gated_scene():
+    from fastsar_amd import gate
+    col, S, grid = wide_band()
+    return col, S, grid, gate.gate_plan(col.ant, None, col.fmin, col.df, col.K, **grid)
+
+
+def _gate_images(col, S, grid, g, Sg, win):
+    pts = oracle.plane_points(**grid)
+    Sw = S if win is None else S * (win[0][:, None] * win[1][None, :])
+    full = oracle.bp_points(Sw, col.ant, col.fmin, col.df, pts, upsample=16, interp='cubic')
+    gated = oracle.bp_points(Sg, col.ant, g['fmin'], g['df'], pts, upsample=16, interp='cubic')
+    return gated, full
+
+
+@pytest.mark.parametrize('window', [False, True])
+def test_range_gate_image(gated_scene, window):
+    """The gated history forms the image of the whole one: float64 exact backprojection of both onto the image's
+    grid, compared with no fitted gain, so that the range cut, the direction of either transform and the Kp / N
+    scale are all pinned (the reference's criterion for its formers is -55 dB, tests/test_api.py:151-167; with
+    exact backprojection the gate alone is measured: -98 dB unwindowed, -110 dB windowed)."""
+    col, S, grid, g = gated_scene
+    win = oracle.window(col.Np, col.K) if window else None
+    gated, full = _gate_images(col, S, grid, g, oracle.range_gate(S, g, win), win)
+    e = db(gated, full)
+    print(f'range gate, window {window}: {g["k1"] - g["k0"]} of {col.K} samples, {e:.1f} dB')
+    assert e < -85
+
+
+def test_range_gate_planted_mistakes(gated_scene):
+    """Each plausible mistake in the gate fails the image check: the range cut kept as one block (no negative
+    ranges), the transforms swapped, the Kp / N scale dropped."""
+    col, S, grid, g = gated_scene
+    npl, Kp, N, k0, k1 = g['npl'], g['Kp'], g['N'], g['k0'], g['k1']
+    z = np.zeros((col.Np, Kp), np.complex128)
+    z[:, npl:npl + col.K] = S
+    q = np.fft.ifft(z, axis=1)
+    h = N // 2
+    folded = np.concatenate([q[:, :N - h], q[:, Kp - h:]], axis=1)
+    wrong = {
+        'one block': np.fft.fft(q[:, :N], axis=1)[:, k0:k1] * g['taper'],
+        'swapped transforms': np.fft.ifft(np.fft.fft(z, axis=1)[:, list(range(N - h)) + list(range(Kp - h, Kp))], axis=1)[:, k0:k1] * g['taper'] * N * N / Kp,
+        'no scale': np.fft.fft(folded, axis=1)[:, k0:k1] * g['taper'] * (N / Kp),
+    }
+    for name, Sg in wrong.items():
+        gated, full = _gate_images(col, S, grid, g, Sg, None)
+        e = db(gated, full)
+        print(f'{name}: {e:.1f} dB')
+        assert e > -20, name
+
+
+def test_range_gate_plan():
+    """The plan's shape: a narrow image of a wide-band history keeps few samples; the taper is 1 over the band and
+    falls to 0 across the guards; an image that fills the swath is not worth gating (the reference's rule: gated
+    only when N < Kp and fewer than half the samples are kept, fastsar/api.py:409-411)."""
+    from fastsar_amd import gate
+    col, _, grid = wide_band(ncl=1)
+    g = gate.gate_plan(col.ant, None, col.fmin, col.df, col.K, **grid)
+    kept = g['k1'] - g['k0']
+    assert g['N'] < g['Kp'] and kept < col.K // 20 and g['N'] % 2 == 0
+    f = g['fmin'] + g['df'] * np.arange(kept)
+    band = (col.fmin, col.fmin + (col.K - 1) * col.df)
+    inside = (f >= band[0]) & (f <= band[1])
+    assert np.allclose(g['taper'][inside], g['Kp'] / g['N'], rtol=1e-6)
+    assert np.all(g['taper'][~inside] < g['Kp'] / g['N']) and np.all(g['taper'] > 0)
+    assert f[0] >= band[0] - 0.125 * (band[1] - band[0]) - 1e-3 and f[-1] <= band[1] + 0.125 * (band[1] - band[0]) + 1e-3
+    small = sim.make_collect(res=0.5, scene=60.0, r0=5e3)
+    gs = gate.gate_plan(small.ant, None, small.fmin, small.df, small.K, 256, 256, 0.25, 0.25, (1.0, 0, 0), (0, 1.0, 0))
+    assert not (gs['N'] < gs['Kp'] and gs['k1'] - gs['k0'] < small.K // 2)

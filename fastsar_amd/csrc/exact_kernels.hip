@@ -52,21 +52,67 @@ __global__ __launch_bounds__(256) void k_pad_window(const float2* __restrict__ s
 }
 
 // win[p, m] = prof[p, (blo[p] + m - nfft/2) mod nfft] * scale: the W reachable bins of each profile, the fftshift
-// folded into the index (fastsar/exact.py:197-205)
+// folded into the index (fastsar/exact.py:197-205). With smax, the largest magnitude of the chunk's window as well
+// (float bits compared as unsigned integers, which orders non-negative floats; one atomic per wave).
 __global__ __launch_bounds__(256) void k_crop(const float2* __restrict__ prof, const long long* __restrict__ blo, int n,
-                                               int nfft, int W, float scale, float2* __restrict__ win)
+                                               int nfft, int W, float scale, float2* __restrict__ win,
+                                               unsigned* __restrict__ smax)
 {
     const size_t total = (size_t)n * W;
+    float mx = 0.f;
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
         const int p = (int)(idx / W), m = (int)(idx - (size_t)p * W);
         long long src = blo[p] + m - nfft / 2;
         src = ((src % nfft) + nfft) % nfft;
         const float2 v = prof[(size_t)p * nfft + src];
-        win[idx] = make_float2(v.x * scale, v.y * scale);
+        const float2 o = make_float2(v.x * scale, v.y * scale);
+        win[idx] = o;
+        mx = fmaxf(mx, sqrtf(o.x * o.x + o.y * o.y));
+    }
+    if (smax) {
+        for (int k = 32; k >= 1; k >>= 1) mx = fmaxf(mx, __shfl_xor(mx, k, 64));
+        if ((threadIdx.x & 63) == 0 && mx > 0.f) atomicMax(smax, __float_as_uint(mx));
+    }
+}
+
+// float16 profiles: the chunk's window scaled so that its largest magnitude is 2048 (inside the float16 range with
+// a margin for the interpolation) and rounded to nearest (fastsar/exact.py:164-184). PAIR: word m of a row holds
+// samples m and m + 1 (the last word repeats the last sample), so that a linear read is one 8-byte load.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float2 h2f(half2_t h) { return make_float2((float)h.x, (float)h.y); }
+
+__device__ __forceinline__ float profile_scale(const unsigned* smax) { return 2048.f / fmaxf(__uint_as_float(*smax), 1e-30f); }
+
+template <bool PAIR>
+__global__ __launch_bounds__(256) void k_to_half(const float2* __restrict__ win, int n, int W, const unsigned* __restrict__ smax,
+                                                  half2_t* __restrict__ out)
+{
+    const float sc = profile_scale(smax);
+    const size_t total = (size_t)n * W;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const float2 v = win[idx];
+        const half2_t a = {(_Float16)(v.x * sc), (_Float16)(v.y * sc)};
+        if (PAIR) {
+            const int m = (int)(idx % W);
+            const float2 u = win[idx + (m + 1 < W ? 1 : 0)];
+            const half2_t b = {(_Float16)(u.x * sc), (_Float16)(u.y * sc)};
+            out[2 * idx] = a;
+            out[2 * idx + 1] = b;
+        } else {
+            out[idx] = a;
+        }
     }
 }
 
 // ------------------------------------------------------------------------------------------------ bp_tiles
+
+template <int PT>
+__device__ __forceinline__ float2 ld_prof(const void* w, size_t i)
+{
+    if (PT == 0) return static_cast<const float2*>(w)[i];
+    return h2f(static_cast<const half2_t*>(w)[i]);
+}
 
 // One workgroup of 256 threads: NTL tiles of TX x TY pixels (T = TX TY; NTL = max(1, 256 / T)), PX = max(1, T / 256)
 // pixels per thread. Per batch of PB = 256 / NTL pulses each thread computes one (tile, pulse) centre term in
@@ -74,8 +120,11 @@ __global__ __launch_bounds__(256) void k_crop(const float2* __restrict__ prof, c
 // at offset d from the centre: du = d.u, del = du + (|d|^2 - du^2)(1/(2r) - du/(2r^2)), bin t = tf + del inv_dr,
 // read cubic (or linear), phase ph0 + kcyc del. The image accumulates over the chunks of a call (out +=), each pixel
 // owned by one thread, in pulse order. (fastsar/exact.py:71-162)
-template <int TX, int TY>
-__global__ __launch_bounds__(256) void k_bp_tiles(const float2* __restrict__ win, int P, int W, const long long* __restrict__ blo,
+// PT: the profiles' storage, 0 float32 (float2), 1 float16 (half2), 2 float16 pairs (two half2 per sample: m, m + 1).
+// With float16 the chunk's scale (smax / 2048) is undone once per pixel when the chunk's sum is added to the image.
+template <int TX, int TY, int PT>
+__global__ __launch_bounds__(256) void k_bp_tiles(const void* __restrict__ winv, const unsigned* __restrict__ smax, int P, int W,
+                                                   const long long* __restrict__ blo,
                                                    int nfft, const double* __restrict__ ant, const double* __restrict__ ref,
                                                    const double* __restrict__ cen, int ntx, int nty, int nx, int ny, float sx,
                                                    float sy, float e1x, float e1y, float e1z, float e2x, float e2y, float e2z,
@@ -130,7 +179,7 @@ __global__ __launch_bounds__(256) void k_bp_tiles(const float2* __restrict__ win
             for (int k = 0; k < np; ++k) {
                 const float ux = s_ux[tl][k], uy = s_uy[tl][k], uz = s_uz[tl][k], hir = s_ir[tl][k], hi2 = s_i2[tl][k];
                 const float tfk = s_tf[tl][k], phk = s_ph[tl][k];
-                const float2* row = win + (size_t)(p0 + k) * W;
+                const size_t row0 = (size_t)(p0 + k) * W;
                 const int ti = s_ti[tl][k];
                 #pragma unroll
                 for (int m = 0; m < PX; ++m) {
@@ -139,15 +188,24 @@ __global__ __launch_bounds__(256) void k_bp_tiles(const float2* __restrict__ win
                     const float t = tfk + del * fdr, fl = floorf(t), w = t - fl;
                     const int i = min(max(ti + (int)fl, 1), W - 3);
                     float vre, vim;
-                    if (cubic) {
-                        const float2 a0 = row[i - 1], a1 = row[i], a2 = row[i + 1], a3 = row[i + 2];
+                    if (PT != 2 && cubic) {
+                        const float2 a0 = ld_prof<PT>(winv, row0 + i - 1), a1 = ld_prof<PT>(winv, row0 + i);
+                        const float2 a2 = ld_prof<PT>(winv, row0 + i + 1), a3 = ld_prof<PT>(winv, row0 + i + 2);
                         const float wm1 = w - 1.f, wm2 = w - 2.f, wp1 = w + 1.f;
                         const float c0 = -w * wm1 * wm2 * (1.f / 6.f), c1 = wp1 * wm1 * wm2 * 0.5f;
                         const float c2 = -wp1 * w * wm2 * 0.5f, c3 = wp1 * w * wm1 * (1.f / 6.f);
                         vre = c0 * a0.x + c1 * a1.x + c2 * a2.x + c3 * a3.x;
                         vim = c0 * a0.y + c1 * a1.y + c2 * a2.y + c3 * a3.y;
                     } else {
-                        const float2 r0 = row[i], r1 = row[i + 1];
+                        float2 r0, r1;
+                        if (PT == 2) {
+                            const half2_t* hp = static_cast<const half2_t*>(winv) + 2 * (row0 + i);
+                            r0 = h2f(hp[0]);
+                            r1 = h2f(hp[1]);
+                        } else {
+                            r0 = ld_prof<PT>(winv, row0 + i);
+                            r1 = ld_prof<PT>(winv, row0 + i + 1);
+                        }
                         vre = fmaf(w, r1.x - r0.x, r0.x);
                         vim = fmaf(w, r1.y - r0.y, r0.y);
                     }
@@ -163,13 +221,14 @@ __global__ __launch_bounds__(256) void k_bp_tiles(const float2* __restrict__ win
         __syncthreads();
     }
     if (!bok) return;
+    const float us = PT ? 1.f / profile_scale(smax) : 1.f;
     #pragma unroll
     for (int m = 0; m < PX; ++m)
         if (ix[m] < nx && iy[m] < ny) {
             const size_t o = (size_t)ix[m] * ny + iy[m];
             float2 v = out[o];
-            v.x += ar[m];
-            v.y += ai[m];
+            v.x = fmaf(ar[m], us, v.x);
+            v.y = fmaf(ai[m], us, v.y);
             out[o] = v;
         }
 }
@@ -234,6 +293,8 @@ struct Exact {
     float2* pad = nullptr;           // [chunk, nfft]
     float2* prof = nullptr;          // [chunk, nfft]
     float2* win = nullptr;           // [chunk, W]
+    void* winh = nullptr;            // [chunk, W] float16 profiles (half2, or two half2 per sample in pairs)
+    unsigned* smax = nullptr;        // the chunk's largest profile magnitude (float bits)
     float2* image = nullptr;
     int* bad = nullptr;
     hipfftHandle plan = 0;
@@ -244,7 +305,7 @@ struct Exact {
     {
         if (have_plan) hipfftDestroy(plan);
         dev_free(ant); dev_free(ref); dev_free(cen); dev_free(blo); dev_free(wp); dev_free(wk); dev_free(hist);
-        dev_free(pad); dev_free(prof); dev_free(win); dev_free(image); dev_free(bad);
+        dev_free(pad); dev_free(prof); dev_free(win); dev_free(winh); dev_free(smax); dev_free(image); dev_free(bad);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -260,7 +321,8 @@ static int upload_e(T** dst, const T* src, size_t count)
     return OK;
 }
 
-static int launch_bp_tiles(const Exact& e, const float2* win, int n, int p0, float2* out)
+template <int PT>
+static int launch_bp_tiles_t(const Exact& e, const void* win, int n, int p0, float2* out)
 {
     const fsh_exact_desc& d = e.d;
     const int ntiles = d.ntx * d.nty;
@@ -268,10 +330,10 @@ static int launch_bp_tiles(const Exact& e, const float2* win, int n, int p0, flo
 #define FSH_BP(TX_, TY_)                                                                                              \
     do {                                                                                                              \
         constexpr int NTL = (TX_ * TY_) >= 256 ? 1 : 256 / (TX_ * TY_);                                               \
-        hipLaunchKernelGGL((k_bp_tiles<TX_, TY_>), dim3((ntiles + NTL - 1) / NTL), dim3(256), 0, e.stream, win, n, d.W, \
-                           e.blo + p0, d.nfft, e.ant + 3 * (size_t)p0, e.ref + p0, e.cen, d.ntx, d.nty, d.nx, d.ny, sx, sy,  \
-                           (float)d.e1[0], (float)d.e1[1], (float)d.e1[2], (float)d.e2[0], (float)d.e2[1], (float)d.e2[2], \
-                           d.inv_dr, d.kcyc, d.cubic, out);                                                           \
+        hipLaunchKernelGGL((k_bp_tiles<TX_, TY_, PT>), dim3((ntiles + NTL - 1) / NTL), dim3(256), 0, e.stream, win, e.smax, \
+                           n, d.W, e.blo + p0, d.nfft, e.ant + 3 * (size_t)p0, e.ref + p0, e.cen, d.ntx, d.nty, d.nx, d.ny, \
+                           sx, sy, (float)d.e1[0], (float)d.e1[1], (float)d.e1[2], (float)d.e2[0], (float)d.e2[1],      \
+                           (float)d.e2[2], d.inv_dr, d.kcyc, d.cubic, out);                                            \
     } while (0)
     if (d.tx == 32 && d.ty == 32) FSH_BP(32, 32);
     else if (d.tx == 16 && d.ty == 16) FSH_BP(16, 16);
@@ -287,6 +349,15 @@ static int launch_bp_tiles(const Exact& e, const float2* win, int n, int p0, flo
     return OK;
 }
 
+static int launch_bp_tiles(const Exact& e, int n, int p0, float2* out)
+{
+    switch (e.d.profiles) {
+    case 1: return launch_bp_tiles_t<1>(e, e.winh, n, p0, out);
+    case 2: return launch_bp_tiles_t<2>(e, e.winh, n, p0, out);
+    default: return launch_bp_tiles_t<0>(e, e.win, n, p0, out);
+    }
+}
+
 }  // namespace fsh
 
 using namespace fsh;
@@ -299,7 +370,8 @@ int fsh_exact_create(const fsh_exact_desc* desc, void** handle)
     *handle = nullptr;
     const fsh_exact_desc& d = *desc;
     if (d.P < 1 || d.K < 2 || d.nx < 1 || d.ny < 1 || d.nfft < d.K || d.W < 8 || d.W > d.nfft + 8 || d.chunk < 1 ||
-        d.ntx < 1 || d.nty < 1 || !d.ant || !d.ref || !d.cen || !d.blo) {
+        d.ntx < 1 || d.nty < 1 || !d.ant || !d.ref || !d.cen || !d.blo || d.profiles < 0 || d.profiles > 2 ||
+        (d.profiles == 2 && d.cubic)) {
         set_error("exact_create: invalid descriptor");
         return E_INVALID;
     }
@@ -320,6 +392,9 @@ int fsh_exact_create(const fsh_exact_desc* desc, void** handle)
         (st = dev_alloc((void**)&e->prof, (size_t)ch * d.nfft * sizeof(float2))) ||
         (st = dev_alloc((void**)&e->win, (size_t)ch * d.W * sizeof(float2))) ||
         (st = dev_alloc((void**)&e->image, (size_t)d.nx * d.ny * sizeof(float2))) || (st = dev_alloc((void**)&e->bad, sizeof(int))))
+        return fail(st);
+    if (d.profiles && ((st = dev_alloc(&e->winh, (size_t)ch * d.W * (d.profiles == 2 ? 8 : 4))) ||
+                       (st = dev_alloc((void**)&e->smax, sizeof(unsigned)))))
         return fail(st);
     if (hipMemset(e->pad, 0, (size_t)ch * d.nfft * sizeof(float2)) != hipSuccess) return fail(E_RUNTIME);
     *handle = e;
@@ -360,10 +435,16 @@ int fsh_exact_form(void* handle, const void* S, int S_on_device, void* image, in
                              "hipfftExecC2C")))
             return st;
         const size_t tw = (size_t)n * d.W;
-        hipLaunchKernelGGL(k_crop, dim3((unsigned)std::min<size_t>((tw + 255) / 256, 1 << 16)), dim3(256), 0, e->stream, e->prof,
-                           e->blo + p0, n, d.nfft, d.W, 1.0f, e->win);
+        const dim3 gw((unsigned)std::min<size_t>((tw + 255) / 256, 1 << 16));
+        if (d.profiles && (st = launch_fill_int((int*)e->smax, 1, 0, e->stream))) return st;
+        hipLaunchKernelGGL(k_crop, gw, dim3(256), 0, e->stream, e->prof, e->blo + p0, n, d.nfft, d.W, 1.0f, e->win, e->smax);
         FSH_TRY(hipGetLastError());
-        if ((st = launch_bp_tiles(*e, e->win, n, p0, e->image))) return st;
+        if (d.profiles == 1)
+            hipLaunchKernelGGL(k_to_half<false>, gw, dim3(256), 0, e->stream, e->win, n, d.W, e->smax, (half2_t*)e->winh);
+        else if (d.profiles == 2)
+            hipLaunchKernelGGL(k_to_half<true>, gw, dim3(256), 0, e->stream, e->win, n, d.W, e->smax, (half2_t*)e->winh);
+        FSH_TRY(hipGetLastError());
+        if ((st = launch_bp_tiles(*e, n, p0, e->image))) return st;
     }
     int badh = INT_MAX;
     FSH_TRY(hipMemcpyAsync(&badh, e->bad, sizeof(int), hipMemcpyDeviceToHost, e->stream));

@@ -31,7 +31,7 @@ class ExactDesc(ctypes.Structure):
                [(n, ctypes.c_double) for n in ('spx', 'spy', 'inv_dr', 'kcyc')] + \
                [('e1', ctypes.c_double * 3), ('e2', ctypes.c_double * 3),
                 ('ant', _hip.P_f64), ('ref', _hip.P_f64), ('cen', _hip.P_f64), ('blo', ctypes.POINTER(ctypes.c_longlong)),
-                ('wp', _hip.P_f32), ('wk', _hip.P_f32)]
+                ('wp', _hip.P_f32), ('wk', _hip.P_f32), ('profiles', ctypes.c_int)]
 
 
 def _bind():
@@ -103,9 +103,15 @@ class ExactFormer:
         from .api import _backend, _check_positions, _check_grid, _window
         if interp not in ('cubic', 'linear'):
             raise ValueError(f"interp must be 'cubic' or 'linear', got {interp!r}")
-        if profiles not in ('auto', 'float32'):
-            raise ValueError(f"profiles must be 'auto' or 'float32' on the hip backend, got {profiles!r}")
-        self.profiles = 'float32'
+        if profiles not in ('auto', 'float32', 'float16', 'float16-pair'):
+            raise ValueError(f"profiles must be 'auto', 'float32', 'float16' or 'float16-pair', got {profiles!r}")
+        if profiles == 'float16-pair' and interp != 'linear':
+            raise ValueError("profiles='float16-pair' applies to interp='linear'")
+        # 'auto' is float32 here. The reference picks float16 for linear reads on grids of 2^25 pixels and more,
+        # where its pixel gather dominates (fastsar/exact.py:420-430); on gfx950 the gather is served from cache and
+        # the kernel is bound by its arithmetic, to which the conversions add: a 4096^2 grid from 2048 pulses forms
+        # in 40.5 ms with float32 profiles and in 52.0 (float16) or 50.4 ms (pairs) with float16 ones.
+        self.profiles = 'float32' if profiles == 'auto' else profiles
         self.ant = _check_positions(ant)
         self.P = len(self.ant)
         self.K = _integer(K, 'K (frequency samples per pulse)', 2)
@@ -210,6 +216,7 @@ class ExactFormer:
             d.P, d.K, d.nx, d.ny, d.tx, d.ty = self.P, self.K, self.nx, self.ny, self.tx, self.ty
             d.ntx, d.nty, d.nfft, d.W, d.chunk, d.cubic = self.ntx, self.nty, self.nfft, self.W, min(self.chunk, self.P), int(self.cubic)
             d.spx, d.spy, d.inv_dr, d.kcyc = self.spx, self.spy, self.inv_dr, self.kcyc
+            d.profiles = {'float32': 0, 'float16': 1, 'float16-pair': 2}[self.profiles]
             for k in range(3):
                 d.e1[k], d.e2[k] = self.e1[k], self.e2[k]
             keep = [np.ascontiguousarray(self.ant), np.ascontiguousarray(self.ref), self.cen, self.blo]
@@ -229,7 +236,8 @@ class ExactFormer:
         """dict(backend, needed, available, full_speed, parts) in bytes. (fastsar/exact.py:736-761)"""
         ch = min(self.chunk, self.P)
         parts = dict(history=self.P * self.K * 8, padded=ch * self.nfft * 8, fft=ch * self.nfft * 8,
-                     window=ch * self.W * 8, image=self.nx * self.ny * 8, geometry=self.cen.nbytes + self.P * 40)
+                     window=ch * self.W * {'float32': 8, 'float16': 12, 'float16-pair': 16}[self.profiles],
+                     image=self.nx * self.ny * 8, geometry=self.cen.nbytes + self.P * 40)
         need = float(sum(parts.values()))
         avail = float(_hip.device_info()['mem'])
         return dict(backend=self.backend, needed=need, available=avail, full_speed=need <= avail, parts=parts)

@@ -168,6 +168,8 @@ typedef struct fsh_exact_desc {
     const long long* blo;          /* [P] */
     const float* wp;               /* [P] or NULL */
     const float* wk;               /* [K] or NULL */
+    int profiles;                  /* 0 float32; 1 float16, 2 float16 pairs (linear only): the cropped profiles of a
+                                      chunk scaled to peak 2048 and rounded (fastsar/exact.py:164-184, 420-430) */
 } fsh_exact_desc;
 
 int fsh_exact_create(const fsh_exact_desc* desc, void** handle);

@@ -73,10 +73,12 @@ def test_stage_shape_validation(lib):
 
 
 def test_plan_descriptor_layout(tmp_path):
-    """the ctypes mirrors of fsh_level_desc, fsh_plan_desc and fsh_exact_desc have the layout a C compiler gives
-    the header's structs (every field's offset and the sizes)"""
+    """the ctypes mirrors of fsh_level_desc, fsh_plan_desc, fsh_exact_desc and fsh_gate_desc have the layout a C
+    compiler gives the header's structs (every field's offset and the sizes)"""
     from fastsar_amd.exact import ExactDesc
-    structs = [('fsh_level_desc', _hip.LevelDesc), ('fsh_plan_desc', _hip.PlanDesc), ('fsh_exact_desc', ExactDesc)]
+    from fastsar_amd.gate import GateDesc
+    structs = [('fsh_level_desc', _hip.LevelDesc), ('fsh_plan_desc', _hip.PlanDesc), ('fsh_exact_desc', ExactDesc),
+               ('fsh_gate_desc', GateDesc)]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fsh.h"', 'int main(void) {']
     for cname, cls in structs:
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

@@ -70,6 +70,42 @@ def test_exact_former_chunks_and_device_io(scene):
     assert np.array_equal(out.to_host(), f(S))
 
 
+@pytest.mark.parametrize('profiles,interp', [('float16', 'linear'), ('float16-pair', 'linear'), ('float16', 'cubic')])
+def test_exact_float16_profiles(scene, profiles, interp):
+    """float16 profiles (each chunk scaled to peak 2048) against float32 ones: the rounding (about -78 dB) stays
+    below -60 dB (the reference's bound, tests/test_pfa_paths.py:280-291); with linear reads it is far below the
+    interpolation error (the error against the FP64 oracle moves by less than 1 dB), with cubic reads it becomes the
+    larger term (-76 dB against -81 dB in float32) and the image keeps the cubic bound, -65 dB
+    (tests/test_exact.py:40-50). Chunks of 37 pulses, so that each chunk has its own scale."""
+    r0, col, S, ref = scene
+    kw = dict(interp=interp, chunk=37)
+    a = ExactFormer(col.ant, col.fmin, col.df, S.shape[1], **GRID, profiles='float32', **kw)(S)
+    f = ExactFormer(col.ant, col.fmin, col.df, S.shape[1], **GRID, profiles=profiles, **kw)
+    assert f.profiles == profiles
+    b = f(S)
+    e, ea, eb = oracle.rel_db(b, a), oracle.rel_db(a, ref), oracle.rel_db(b, ref)
+    print(f'{profiles} {interp} at {r0 / 1e3:.0f} km: {e:.1f} dB against float32 profiles; oracle {eb:.1f} (float32 '
+          f'{ea:.1f}) dB')
+    assert e < -60 and (eb < ea + 1.0 if interp == 'linear' else eb < -65), (e, ea, eb)
+    assert abs(np.vdot(b, a) / np.vdot(a, a) - 1) < 1e-3          # the scale is undone exactly
+
+
+def test_exact_profiles_choice():
+    """'auto' is float32 on this backend, also for linear reads on grids of 2^25 pixels or more, where the reference
+    picks float16 for its cuda kernel (fastsar/exact.py:420-430: float16 profiles are slower on gfx950); float16
+    pairs are for linear reads only"""
+    wide = sim.spotlight(64, 16384, 3.0, r0=20e3)                   # 8.2 km unambiguous range: a 4 km x 2 km grid fits
+    big = dict(GRID, nx=8192, ny=4096)
+    assert ExactFormer(wide.ant, wide.fmin, wide.df, wide.K, **big, interp='linear').profiles == 'float32'
+    assert ExactFormer(wide.ant, wide.fmin, wide.df, wide.K, **big, interp='linear', profiles='float16').profiles == 'float16'
+    col = sim.make_collect(res=0.5, scene=60.0, r0=20e3)
+    assert ExactFormer(col.ant, col.fmin, col.df, col.K, **GRID, interp='linear').profiles == 'float32'
+    with pytest.raises(ValueError, match='profiles'):
+        ExactFormer(col.ant, col.fmin, col.df, col.K, **GRID, profiles='float8')
+    with pytest.raises(ValueError, match='float16-pair'):
+        ExactFormer(col.ant, col.fmin, col.df, col.K, **GRID, profiles='float16-pair')
+
+
 def test_exact_former_rejects_nonfinite():
     col, _, _, S = _scene(20e3, n=3)
     S = S.copy()

@@ -122,6 +122,8 @@ def main():
     ap.add_argument('--no-profile', action='store_true', help='timed steps only (for counter runs under rocprofv3)')
     ap.add_argument('--calibrate-copy', action='store_true', help='also run the plain copy kernel on 1 GiB three times')
     ap.add_argument('--small', action='store_true', help='the 512 x 512 scene instead (quick check)')
+    ap.add_argument('--history', default='complex64', choices=['complex64', 'float16', 'bfloat16'],
+                    help='storage of the phase history the former reads (float16: per-pulse power-of-two scaled)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU: --gpus {args.gpus} is not supported (the image-formation path needs no '
@@ -141,6 +143,12 @@ def main():
     t0 = time.perf_counter()
     dS = simulate_device(col, pos, amp, clutter_sigma=sim.clutter_sigma(amp, cl_db), seed=seed)
     log(f'phase history {col.Np} x {col.K} generated on the device in {time.perf_counter() - t0:.2f} s')
+    hist, hist_host = dS, None
+    if args.history != 'complex64':
+        from fastsar_amd import precision as pr
+        hist_host = pr.pack(dS.to_host(), args.history)
+        hist = hist_host.to_device()
+        log(f'history stored as {args.history} pairs ({hist_host.data.nbytes / 1e9:.2f} GB)')
 
     t0 = time.perf_counter()
     former = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T, levels=args.levels or None,
@@ -153,12 +161,12 @@ def main():
             f'[{lv["Po"]} x {lv["Ko"]}], Lk={lv["fir_k"]["L"] if lv["fir_k"] else 1}, Lp={lv["fir_p"]["L"] if lv["fir_p"] else 1}')
     img = _hip.DeviceArray((grid['nx'], grid['ny']), np.complex64)
     for i in range(args.warmup):
-        former(dS, out=img)
+        former(hist, out=img)
     _hip.check(_hip.lib().fsh_synchronize())
     log(f'warm: group of {former._form.info()["group"]} first-level children')
     t0 = time.perf_counter()
     for i in range(args.steps):
-        former(dS, out=img)
+        former(hist, out=img)
     _hip.check(_hip.lib().fsh_synchronize())
     dt = (time.perf_counter() - t0) / args.steps
     pairs = grid['nx'] * grid['ny'] * col.Np
@@ -167,7 +175,7 @@ def main():
     # host to host (pageable history in, image out): the PCIe-inclusive figure, reported beside the device rate
     h2h = None
     if args.host_steps > 0 and not args.no_profile:
-        S_host = dS.to_host()
+        S_host = dS.to_host() if hist_host is None else hist_host
         former(S_host)
         t0 = time.perf_counter()
         for i in range(args.host_steps):
@@ -189,7 +197,7 @@ def main():
 
     # per-launch device times (events on the former's stream) of one more call
     former._form.set_profile(True)
-    former(dS, out=img)
+    former(hist, out=img)
     prof = former._form.profile()
     former._form.set_profile(False)
     agg = {}
@@ -247,7 +255,8 @@ def main():
                warmup=args.warmup, ms_per_step=round(dt * 1e3, 2), higher_is_better=True, scaling='weak',
                vs_baseline=None, dtype='f32', data='synthetic',
                config=dict(workload=(f'{"512x512 from 512x1024" if args.small else "12288x8192 image from 16384 pulses x 8192 samples"}'
-                                     f' complex64, factorized backprojection, T={former.T}, {former.levels} levels, {args.atten:g} dB filters'),
+                                     f' {args.history}, factorized backprojection, T={former.T}, {former.levels} levels, {args.atten:g} dB filters'),
+                           history=args.history,
                            s_per_image=round(dt, 4), error_db=round(err, 2), error_tiles=f'{len(tiles)} x {side}^2',
                            predicted_error_db=round(former.predicted_error_db, 2), parallelism='single GPU',
                            host_to_host_ms=None if h2h is None else round(h2h * 1e3, 1)),

@@ -249,9 +249,15 @@ class ImageFormer:
         return S
 
     def __call__(self, S, out=None):
-        """S [P, K] complex phase history (numpy, or a fastsar_amd._hip.DeviceArray) -> complex64 image [nx, ny].
-        The window and the scan for NaN and inf run on the device. out: an optional DeviceArray for the image."""
+        """S [P, K] complex phase history (numpy, a fastsar_amd._hip.DeviceArray, or a 16-bit history from
+        fastsar_amd.precision.pack) -> complex64 image [nx, ny]. The decode, the window and the scan for NaN and inf
+        run on the device. out: an optional DeviceArray for the image."""
         from . import _hip
+        from .precision import History16
+        if isinstance(S, History16):
+            if self._gate is not None:
+                raise ValueError('a 16-bit history is formed ungated: pass it as complex64 to a gated former')
+            return self._form.form16(S, out=out)
         if not isinstance(S, _hip.DeviceArray):
             S = np.asarray(S)
             if S.shape != (self.P, self.K):

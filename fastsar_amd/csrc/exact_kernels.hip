@@ -18,6 +18,8 @@
 
 namespace fsh {
 
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
 static int fft_status(hipfftResult r, const char* what)
 {
     if (r == HIPFFT_SUCCESS) return OK;
@@ -29,15 +31,30 @@ static int fft_status(hipfftResult r, const char* what)
 
 // rows [n, K] of the history (pulses p0 + i) windowed and written centred into pad [n, nfft]: sample k >= K/2 to
 // bin k - K/2, k < K/2 to bin nfft - K/2 + k (the bins between stay zero)
-template <class TOUT>
-__global__ __launch_bounds__(256) void k_pad_window(const float2* __restrict__ s, const float* __restrict__ wp,
+// DT: the history's type, 0 complex64, 1 float16 pairs times inv[p] (pulse p0 + i of the whole history), 2 bfloat16
+// pairs (the 32-bit words of fsh_ffbp_form_typed)
+template <int DT>
+__device__ __forceinline__ float2 load_sample(const void* s, size_t idx, const float* inv, int p)
+{
+    if (DT == 0) return static_cast<const float2*>(s)[idx];
+    const unsigned w = static_cast<const unsigned*>(s)[idx];
+    if (DT == 1) {
+        const half2_t h = __builtin_bit_cast(half2_t, w);
+        const float g = inv[p];
+        return make_float2((float)h.x * g, (float)h.y * g);
+    }
+    return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+}
+
+template <class TOUT, int DT = 0>
+__global__ __launch_bounds__(256) void k_pad_window(const void* __restrict__ s, const float* __restrict__ wp,
                                                      const float* __restrict__ wk, int n, int K, int nfft, TOUT* __restrict__ pad,
-                                                     int* __restrict__ bad, int p0)
+                                                     int* __restrict__ bad, int p0, const float* __restrict__ inv = nullptr)
 {
     const size_t total = (size_t)n * K;
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
         const int p = (int)(idx / K), k = (int)(idx - (size_t)p * K), h = K / 2;
-        const float2 v = s[idx];
+        const float2 v = load_sample<DT>(s, idx, inv, p0 + p);
         if (bad && !(isfinite(v.x) && isfinite(v.y))) atomicMin(bad, p0 + p);
         const double g = wp ? (double)wp[p] * (double)wk[k] : 1.0;
         TOUT o;
@@ -78,8 +95,6 @@ __global__ __launch_bounds__(256) void k_crop(const float2* __restrict__ prof, c
 // float16 profiles: the chunk's window scaled so that its largest magnitude is 2048 (inside the float16 range with
 // a margin for the interpolation) and rounded to nearest (fastsar/exact.py:164-184). PAIR: word m of a row holds
 // samples m and m + 1 (the last word repeats the last sample), so that a linear read is one 8-byte load.
-typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
-
 __device__ __forceinline__ float2 h2f(half2_t h) { return make_float2((float)h.x, (float)h.y); }
 
 __device__ __forceinline__ float profile_scale(const unsigned* smax) { return 2048.f / fmaxf(__uint_as_float(*smax), 1e-30f); }
@@ -295,6 +310,7 @@ struct Exact {
     float2* win = nullptr;           // [chunk, W]
     void* winh = nullptr;            // [chunk, W] float16 profiles (half2, or two half2 per sample in pairs)
     unsigned* smax = nullptr;        // the chunk's largest profile magnitude (float bits)
+    float* pinv = nullptr;           // a float16 history's per-pulse inverse scales (host input)
     float2* image = nullptr;
     int* bad = nullptr;
     hipfftHandle plan = 0;
@@ -305,7 +321,7 @@ struct Exact {
     {
         if (have_plan) hipfftDestroy(plan);
         dev_free(ant); dev_free(ref); dev_free(cen); dev_free(blo); dev_free(wp); dev_free(wk); dev_free(hist);
-        dev_free(pad); dev_free(prof); dev_free(win); dev_free(winh); dev_free(smax); dev_free(image); dev_free(bad);
+        dev_free(pad); dev_free(prof); dev_free(win); dev_free(winh); dev_free(smax); dev_free(pinv); dev_free(image); dev_free(bad);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -401,17 +417,23 @@ int fsh_exact_create(const fsh_exact_desc* desc, void** handle)
     return OK;
 }
 
-int fsh_exact_form(void* handle, const void* S, int S_on_device, void* image, int image_on_device, int* bad_pulse)
+static int exact_form_impl(Exact* e, const void* S, int dtype, const float* pulse_scale, int S_on_device, void* image,
+                           int image_on_device, int* bad_pulse)
 {
-    if (!handle || !S || !image) { set_error("exact_form: null argument"); return E_INVALID; }
-    Exact* e = static_cast<Exact*>(handle);
     const fsh_exact_desc& d = e->d;
     int st;
-    const float2* src = (const float2*)S;
+    const size_t ssz = dtype == FSH_C64 ? sizeof(float2) : 4;
+    const char* src = static_cast<const char*>(S);
+    const float* inv = pulse_scale;
     if (!S_on_device) {
         if (!e->hist && (st = dev_alloc((void**)&e->hist, (size_t)d.P * d.K * sizeof(float2)))) return st;
-        FSH_TRY(hipMemcpyAsync(e->hist, S, (size_t)d.P * d.K * sizeof(float2), hipMemcpyHostToDevice, e->stream));
-        src = e->hist;
+        FSH_TRY(hipMemcpyAsync(e->hist, S, (size_t)d.P * d.K * ssz, hipMemcpyHostToDevice, e->stream));
+        src = reinterpret_cast<const char*>(e->hist);
+        if (pulse_scale) {
+            if (!e->pinv && (st = dev_alloc((void**)&e->pinv, (size_t)d.P * sizeof(float)))) return st;
+            FSH_TRY(hipMemcpyAsync(e->pinv, pulse_scale, (size_t)d.P * sizeof(float), hipMemcpyHostToDevice, e->stream));
+            inv = e->pinv;
+        }
     }
     const int ch = std::min(d.chunk, d.P);
     if (!e->have_plan) {
@@ -428,8 +450,15 @@ int fsh_exact_form(void* handle, const void* S, int S_on_device, void* image, in
         const int n = std::min(ch, d.P - p0);
         if (n < ch) FSH_TRY(hipMemsetAsync(e->pad + (size_t)n * d.nfft, 0, (size_t)(ch - n) * d.nfft * sizeof(float2), e->stream));
         const size_t tot = (size_t)n * d.K;
-        hipLaunchKernelGGL(k_pad_window<float2>, dim3((unsigned)std::min<size_t>((tot + 255) / 256, 1 << 16)), dim3(256), 0, e->stream,
-                           src + (size_t)p0 * d.K, e->wp ? e->wp + p0 : nullptr, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0);
+        const dim3 gp((unsigned)std::min<size_t>((tot + 255) / 256, 1 << 16));
+        const void* sp = src + (size_t)p0 * d.K * ssz;
+        const float* wpp = e->wp ? e->wp + p0 : nullptr;
+        if (dtype == FSH_C32F16)
+            hipLaunchKernelGGL((k_pad_window<float2, 1>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv);
+        else if (dtype == FSH_C32BF16)
+            hipLaunchKernelGGL((k_pad_window<float2, 2>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv);
+        else
+            hipLaunchKernelGGL((k_pad_window<float2, 0>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv);
         FSH_TRY(hipGetLastError());
         if ((st = fft_status(hipfftExecC2C(e->plan, (hipfftComplex*)e->pad, (hipfftComplex*)e->prof, HIPFFT_BACKWARD),
                              "hipfftExecC2C")))
@@ -459,6 +488,23 @@ int fsh_exact_form(void* handle, const void* S, int S_on_device, void* image, in
         return E_NONFINITE;
     }
     return OK;
+}
+
+int fsh_exact_form(void* handle, const void* S, int S_on_device, void* image, int image_on_device, int* bad_pulse)
+{
+    if (!handle || !S || !image) { set_error("exact_form: null argument"); return E_INVALID; }
+    return exact_form_impl(static_cast<Exact*>(handle), S, FSH_C64, nullptr, S_on_device, image, image_on_device, bad_pulse);
+}
+
+int fsh_exact_form_typed(void* handle, const void* S, int dtype, const float* pulse_scale, int S_on_device, void* image,
+                         int image_on_device, int* bad_pulse)
+{
+    if (!handle || !S || !image) { set_error("exact_form_typed: null argument"); return E_INVALID; }
+    if ((dtype != FSH_C64 && dtype != FSH_C32F16 && dtype != FSH_C32BF16) || ((pulse_scale != nullptr) != (dtype == FSH_C32F16))) {
+        set_error("exact_form_typed: a float16 history goes with per-pulse scales, the other types with none");
+        return E_INVALID;
+    }
+    return exact_form_impl(static_cast<Exact*>(handle), S, dtype, pulse_scale, S_on_device, image, image_on_device, bad_pulse);
 }
 
 int fsh_exact_destroy(void* handle)
@@ -495,7 +541,8 @@ int fsh_bp_points_f64(const float* S, int P, int K, const float* wp, const float
         if (hipMemset(pad, 0, (size_t)ch * nfft * sizeof(double2)) != hipSuccess) { st = E_RUNTIME; break; }
         const size_t tot = (size_t)n * K;
         hipLaunchKernelGGL(k_pad_window<double2>, dim3((unsigned)std::min<size_t>((tot + 255) / 256, 1 << 16)), dim3(256), 0, 0,
-                           (const float2*)S + (size_t)p0 * K, wp ? wp + p0 : nullptr, wk, n, K, nfft, pad, nullptr, p0);
+                           (const void*)((const float2*)S + (size_t)p0 * K), wp ? wp + p0 : nullptr, wk, n, K, nfft, pad,
+                           (int*)nullptr, p0, (const float*)nullptr);
         const hipError_t le = hipGetLastError();
         if (le != hipSuccess) { st = hip_status(le, "pad_window"); break; }
         if ((st = fft_status(hipfftExecZ2Z(plan, (hipfftDoubleComplex*)pad, (hipfftDoubleComplex*)prof, HIPFFT_BACKWARD), "Z2Z")))

@@ -819,6 +819,52 @@ int launch_prepare(const float2* S, int P, int K, const float* wp, const float* 
     return OK;
 }
 
+// 16-bit histories: (Re, Im) as float16 (FSH_C32F16, each pulse stored times a power of two, undone by inv[p]) or
+// bfloat16 (FSH_C32BF16) in one 32-bit word per sample; decoded to float32, windowed and scanned like k_prepare
+template <int DT>
+__global__ __launch_bounds__(256) void k_prepare16(const unsigned* __restrict__ S, int P, int K, const float* __restrict__ inv,
+                                                    const float* __restrict__ wp, const float* __restrict__ wk,
+                                                    float2* __restrict__ x, int* __restrict__ bad)
+{
+    const size_t n = (size_t)P * K;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const unsigned w = S[i];
+        const int p = (int)(i / K);
+        float2 v;
+        if (DT == 1) {
+            typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+            const h2 h = __builtin_bit_cast(h2, w);
+            const float g = inv ? inv[p] : 1.f;
+            v = make_float2((float)h.x * g, (float)h.y * g);
+        } else {
+            v = make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+        }
+        if (!(isfinite(v.x) && isfinite(v.y))) atomicMin(bad, p);
+        if (wp) {
+            const float g = wp[p] * wk[i - (size_t)p * K];
+            v.x *= g; v.y *= g;
+        }
+        x[i] = v;
+    }
+}
+
+int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, const float* wp, const float* wk, float2* x,
+                     int* bad, hipStream_t s)
+{
+    const size_t n = (size_t)P * K;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 1 << 16);
+    if (dtype == 1)
+        hipLaunchKernelGGL(k_prepare16<1>, dim3(blocks), dim3(256), 0, s, (const unsigned*)S, P, K, inv, wp, wk, x, bad);
+    else if (dtype == 2)
+        hipLaunchKernelGGL(k_prepare16<2>, dim3(blocks), dim3(256), 0, s, (const unsigned*)S, P, K, inv, wp, wk, x, bad);
+    else {
+        set_error("prepare: unknown history type");
+        return E_INVALID;
+    }
+    FSH_TRY(hipGetLastError());
+    return OK;
+}
+
 int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
                      int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s)
 {

@@ -129,6 +129,7 @@ struct Former {
     // per call
     float2* hist = nullptr;        // the history as uploaded (host input)
     float2* x = nullptr;           // windowed history
+    float* pinv = nullptr;         // a float16 history's per-pulse inverse scales, uploaded (host input)
     int* bad = nullptr;
     float2* image = nullptr;
     // per group
@@ -168,7 +169,7 @@ struct Former {
         }
         lv.clear();
         dev_free(one); dev_free(wp); dev_free(wk); dev_free(fu); dev_free(fr0); dev_free(fcen); dev_free(tile_xy);
-        dev_free(w0); dev_free(gx); dev_free(gyr); dev_free(hist); dev_free(x); dev_free(bad); dev_free(image);
+        dev_free(w0); dev_free(gx); dev_free(gyr); dev_free(hist); dev_free(x); dev_free(pinv); dev_free(bad); dev_free(image);
         if (ev[0]) (void)hipEventDestroy(ev[0]);
         if (ev[1]) (void)hipEventDestroy(ev[1]);
         if (stream) (void)hipStreamDestroy(stream);
@@ -598,24 +599,34 @@ int fsh_ffbp_stage_ms(void* handle, float* ms, int n)
     return OK;
 }
 
-int fsh_ffbp_form(void* handle, const void* S, int S_on_device, void* image, int image_on_device, int* bad_pulse)
+static int ffbp_form_impl(Former* f, const void* S, int dtype, const float* pulse_scale, int S_on_device, void* image,
+                          int image_on_device, int* bad_pulse)
 {
-    if (!handle || !S || !image) { set_error("ffbp_form: null argument"); return E_INVALID; }
-    Former* f = static_cast<Former*>(handle);
     const fsh_plan_desc& p = f->p;
     int st;
     for (int i = 0; i < ST_N; ++i) f->stage_ms[i] = 0.f;
     f->recs.clear();
     f->cur_level = -1;
-    const float2* src = (const float2*)S;
+    const size_t ssz = dtype == FSH_C64 ? sizeof(float2) : 4;
+    const void* src = S;
+    const float* inv = pulse_scale;
     if (!S_on_device) {
         if (!f->hist && (st = dev_alloc((void**)&f->hist, (size_t)p.P * p.K * sizeof(float2)))) return st;
-        FSH_TRY(hipMemcpyAsync(f->hist, S, (size_t)p.P * p.K * sizeof(float2), hipMemcpyHostToDevice, f->stream));
+        FSH_TRY(hipMemcpyAsync(f->hist, S, (size_t)p.P * p.K * ssz, hipMemcpyHostToDevice, f->stream));
         src = f->hist;
+        if (pulse_scale) {
+            if (!f->pinv && (st = dev_alloc((void**)&f->pinv, (size_t)p.P * sizeof(float)))) return st;
+            FSH_TRY(hipMemcpyAsync(f->pinv, pulse_scale, (size_t)p.P * sizeof(float), hipMemcpyHostToDevice, f->stream));
+            inv = f->pinv;
+        }
     }
     f->mark_begin();
     if ((st = launch_fill_int(f->bad, 1, INT_MAX, f->stream))) return st;
-    if ((st = launch_prepare(src, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream))) return st;
+    if (dtype == FSH_C64) {
+        if ((st = launch_prepare((const float2*)src, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream))) return st;
+    } else if ((st = launch_prepare16(src, dtype, inv, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream))) {
+        return st;
+    }
     f->mark_end(ST_PREP);
     int badh = INT_MAX;
     FSH_TRY(hipMemcpyAsync(&badh, f->bad, sizeof(int), hipMemcpyDeviceToHost, f->stream));
@@ -650,6 +661,27 @@ int fsh_ffbp_form(void* handle, const void* S, int S_on_device, void* image, int
     }
     FSH_TRY(hipStreamSynchronize(f->stream));
     return OK;
+}
+
+int fsh_ffbp_form(void* handle, const void* S, int S_on_device, void* image, int image_on_device, int* bad_pulse)
+{
+    if (!handle || !S || !image) { set_error("ffbp_form: null argument"); return E_INVALID; }
+    return ffbp_form_impl(static_cast<Former*>(handle), S, FSH_C64, nullptr, S_on_device, image, image_on_device, bad_pulse);
+}
+
+int fsh_ffbp_form_typed(void* handle, const void* S, int dtype, const float* pulse_scale, int S_on_device, void* image,
+                        int image_on_device, int* bad_pulse)
+{
+    if (!handle || !S || !image) { set_error("ffbp_form_typed: null argument"); return E_INVALID; }
+    if (dtype != FSH_C64 && dtype != FSH_C32F16 && dtype != FSH_C32BF16) {
+        set_error("ffbp_form_typed: unknown history type");
+        return E_INVALID;
+    }
+    if ((pulse_scale != nullptr) != (dtype == FSH_C32F16)) {
+        set_error("ffbp_form_typed: a float16 history goes with per-pulse scales, the other types with none");
+        return E_INVALID;
+    }
+    return ffbp_form_impl(static_cast<Former*>(handle), S, dtype, pulse_scale, S_on_device, image, image_on_device, bad_pulse);
 }
 
 int fsh_ffbp_destroy(void* handle)

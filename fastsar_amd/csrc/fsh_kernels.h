@@ -6,6 +6,8 @@
 namespace fsh {
 
 int launch_prepare(const float2* S, int P, int K, const float* wp, const float* wk, float2* x, int* bad, hipStream_t s);
+int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, const float* wp, const float* wk, float2* x,
+                     int* bad, hipStream_t s);
 int launch_fill_int(int* p, int n, int value, hipStream_t s);
 int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
                      int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s);

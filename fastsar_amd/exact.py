@@ -45,7 +45,9 @@ def _bind():
                                         ci, vp, ci]
         L.fsh_sim_points.argtypes = [vp, vp, ci, ci, ctypes.c_double, ctypes.c_double, vp, vp, ci, vp, vp]
         L.fsh_add_noise.argtypes = [vp, ctypes.c_size_t, ctypes.c_double, ctypes.c_ulonglong, vp]
-        for n in ('fsh_exact_create', 'fsh_exact_form', 'fsh_exact_destroy', 'fsh_bp_points_f64', 'fsh_sim_points', 'fsh_add_noise'):
+        L.fsh_exact_form_typed.argtypes = [vp, vp, ci, vp, ci, vp, ci, _hip.P_i32]
+        for n in ('fsh_exact_create', 'fsh_exact_form', 'fsh_exact_form_typed', 'fsh_exact_destroy', 'fsh_bp_points_f64',
+                  'fsh_sim_points', 'fsh_add_noise'):
             getattr(L, n).restype = ci
         L._exact_bound = True
     return L
@@ -243,9 +245,21 @@ class ExactFormer:
         return dict(backend=self.backend, needed=need, available=avail, full_speed=need <= avail, parts=parts)
 
     def __call__(self, S, out=None):
-        """S [P, K] complex phase history (numpy or a DeviceArray) -> complex64 image [nx, ny] (or into out, a
-        DeviceArray). NaN and inf are found on the device and raise ValueError naming the first pulse."""
+        """S [P, K] complex phase history (numpy, a DeviceArray, or a 16-bit history from
+        fastsar_amd.precision.pack) -> complex64 image [nx, ny] (or into out, a DeviceArray). NaN and inf are found
+        on the device and raise ValueError naming the first pulse."""
         from .api import _check_history
+        from .precision import History16
+        if isinstance(S, History16):
+            if S.shape != (self.P, self.K):
+                raise ValueError(f'phase history must be {(self.P, self.K)}, got {S.shape}')
+            img_dev = isinstance(out, _hip.DeviceArray)
+            img = out if img_dev else np.empty((self.nx, self.ny), np.complex64)
+            bad = ctypes.c_int(-1)
+            h = self._handle()
+            _hip.check(_bind().fsh_exact_form_typed(h, _hip.ptr(S.data), S.code, _hip.ptr(S.inv), int(S.on_device),
+                                                    _hip.ptr(img), int(img_dev), ctypes.byref(bad)), 'exact_form')
+            return img
         on_dev = isinstance(S, _hip.DeviceArray)
         if not on_dev:
             S = _check_history(np.asarray(S), self.P, finite=False)

@@ -136,6 +136,19 @@ class HipFFBP:
         _hip.check(_hip.lib().fsh_ffbp_stage_ms(self.h, ms, len(STAGES)), 'ffbp_stage_ms')
         return dict(zip(STAGES, [float(v) for v in ms]))
 
+    def form16(self, h, out=None):
+        """h: a History16 (fastsar_amd.precision), host or device, decoded on the device; out as in __call__."""
+        if h.shape != (self.P, self.K):
+            raise ValueError(f'phase history must be {(self.P, self.K)}, got {h.shape}')
+        img_dev = isinstance(out, _hip.DeviceArray)
+        img = out if img_dev else np.empty((self.nx, self.ny), np.complex64)
+        bad = ctypes.c_int(-1)
+        data = h.data if h.on_device else np.ascontiguousarray(h.data, np.uint32)
+        inv = None if h.inv is None else (h.inv if h.on_device else np.ascontiguousarray(h.inv, np.float32))
+        _hip.check(_hip.lib().fsh_ffbp_form_typed(self.h, _hip.ptr(data), h.code, _hip.ptr(inv), int(h.on_device),
+                                                  _hip.ptr(img), int(img_dev), ctypes.byref(bad)), 'ffbp_form')
+        return img
+
     def __call__(self, S, out=None):
         """S [P, K] complex64 (host, C-contiguous) or a DeviceArray; out: None (a new host image) or a DeviceArray
         [nx, ny] complex64 to write into. Returns the image."""

@@ -141,6 +141,15 @@ int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, voi
  * non-finite sample when FSH_ENONFINITE is returned. */
 int fsh_ffbp_form(void* handle, const void* S, int S_on_device, void* image, int image_on_device, int* bad_pulse);
 /* First-level children per group (0: automatic, up to 8, halved while device memory lacks). */
+/* The same with a history in a given type: FSH_C64 complex64 (as fsh_ffbp_form); FSH_C32F16 (Re, Im) float16 pairs,
+ * pulse p stored times a power of two that pulse_scale[p] undoes (its inverse, [P] float32, host or device as S);
+ * FSH_C32BF16 (Re, Im) bfloat16 pairs (pulse_scale NULL). Decoded to float32 on the device before the window. No
+ * reference counterpart: the 16-bit storage of the precision variants (half the upload and history footprint). */
+#define FSH_C64 0
+#define FSH_C32F16 1
+#define FSH_C32BF16 2
+int fsh_ffbp_form_typed(void* handle, const void* S, int dtype, const float* pulse_scale, int S_on_device, void* image,
+                        int image_on_device, int* bad_pulse);
 int fsh_ffbp_set_group(void* handle, int ng);
 /* The group used by the last call and the device bytes the former holds. */
 int fsh_ffbp_info(void* handle, int* ng_used, size_t* device_bytes);
@@ -174,6 +183,9 @@ typedef struct fsh_exact_desc {
 
 int fsh_exact_create(const fsh_exact_desc* desc, void** handle);
 int fsh_exact_form(void* handle, const void* S, int S_on_device, void* image, int image_on_device, int* bad_pulse);
+/* A history in a given type (FSH_C64, FSH_C32F16 with pulse_scale, FSH_C32BF16), as fsh_ffbp_form_typed. */
+int fsh_exact_form_typed(void* handle, const void* S, int dtype, const float* pulse_scale, int S_on_device, void* image,
+                         int image_on_device, int* bad_pulse);
 int fsh_exact_destroy(void* handle);
 
 /* ------------------------------------------------------------------------------------------ range gate

@@ -684,6 +684,40 @@ int fsh_ffbp_form_typed(void* handle, const void* S, int dtype, const float* pul
     return ffbp_form_impl(static_cast<Former*>(handle), S, dtype, pulse_scale, S_on_device, image, image_on_device, bad_pulse);
 }
 
+int fsh_ffbp_enqueue(void* handle, const void* S, void* image, void* stream)
+{
+    if (!handle || !S || !image) { set_error("ffbp_enqueue: null argument"); return E_INVALID; }
+    Former* f = static_cast<Former*>(handle);
+    if (f->ng < 1) {
+        set_error("ffbp_enqueue: the former has no group buffers yet (form one image first, or set_group and form)");
+        return E_INVALID;
+    }
+    if (f->profile) { set_error("ffbp_enqueue: profiling waits on events; switch it off"); return E_INVALID; }
+    const fsh_plan_desc& p = f->p;
+    // the former's launches go to the caller's stream and its final stage writes into the caller's image
+    hipStream_t own = f->stream;
+    float2* own_image = f->image;
+    f->stream = static_cast<hipStream_t>(stream);
+    f->image = static_cast<float2*>(image);
+    int st = launch_fill_int(f->bad, 1, INT_MAX, f->stream);
+    if (!st) st = launch_prepare(static_cast<const float2*>(S), p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream);
+    const int G = f->lv[0].h.C;
+    for (int g0 = 0; !st && g0 < G; g0 += f->ng) st = f->run_group(g0, std::min(f->ng, G - g0));
+    f->stream = own;
+    f->image = own_image;
+    return st;
+}
+
+int fsh_ffbp_bad_pulse(void* handle, int* bad_pulse)
+{
+    if (!handle || !bad_pulse) { set_error("ffbp_bad_pulse: null argument"); return E_INVALID; }
+    Former* f = static_cast<Former*>(handle);
+    int badh = INT_MAX;
+    FSH_TRY(hipMemcpy(&badh, f->bad, sizeof(int), hipMemcpyDeviceToHost));
+    *bad_pulse = badh == INT_MAX ? -1 : badh;
+    return OK;
+}
+
 int fsh_ffbp_destroy(void* handle)
 {
     if (handle) {

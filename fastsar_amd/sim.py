@@ -131,3 +131,33 @@ def config1(seed=1, P=512, K=1024, targets=25, clutter_db=-30.0, side=256.0):
         S = add_noise(S, clutter_sigma(amp, clutter_db), rng)
     grid = dict(nx=512, ny=512, spx=0.5, spy=0.5, e1=(0.0, 1.0, 0.0), e2=(1.0, 0.0, 0.0))
     return col, S.astype(np.complex64), pos, amp, grid
+
+
+def uneven_batch(seed=5, n=48, sides=(256, 4096), pulses=(512, 8192), samples=(1024, 4096), sp=0.5, r0=10e3,
+                 graze_deg=30.0, fc=9.6e9, squint_deg=2.0, targets=20, clutter_db=-30.0, margin=1.25,
+                 max_bandwidth=600e6, max_aperture_deg=6.0):
+    """n images of uneven size for the batched workload: square sides, pulse counts and sample counts drawn
+    log-uniformly (seeded); even-numbered images spotlight, odd ones stripmap-like patches seen at squint_deg. Each
+    collection is sampled for its image: frequency spacing and along-track spacing leave the image's extent (times
+    margin on the diagonal) unambiguous, finer where K or P would otherwise span more than max_bandwidth or
+    max_aperture_deg, so the bandwidth and the aperture follow from K and P. -> list of dict(col, grid, pos, amp,
+    sigma, squint)."""
+    rng = np.random.default_rng(seed)
+    lam = C / fc
+    psi = np.deg2rad(graze_deg)
+    out = []
+    for i in range(n):
+        side = _even(2.0 ** rng.uniform(np.log2(sides[0]), np.log2(sides[1])))
+        P = _even(2.0 ** rng.uniform(np.log2(pulses[0]), np.log2(pulses[1])))
+        K = _even(2.0 ** rng.uniform(np.log2(samples[0]), np.log2(samples[1])))
+        extent = margin * np.sqrt(2.0) * side * sp
+        df = min(C / (2.0 * extent), max_bandwidth / K)
+        dy = min(lam * r0 / (2.0 * extent), 2.0 * r0 * np.cos(psi) * np.tan(np.deg2rad(max_aperture_deg) / 2.0) / (P - 1))
+        half = 0.5 * dy * (P - 1)
+        squint = 0.0 if i % 2 == 0 else squint_deg
+        ap = np.rad2deg(2.0 * np.arctan(half / (r0 * np.cos(psi))))
+        col = spotlight(P, K, ap, r0=r0, graze_deg=graze_deg, fc=fc, bandwidth=K * df, squint_deg=squint)
+        grid = dict(nx=side, ny=side, spx=sp, spy=sp, e1=(0.0, 1.0, 0.0), e2=(1.0, 0.0, 0.0))
+        pos, amp = point_scene(rng, targets, 0.8 * side * sp, 0.8 * side * sp)
+        out.append(dict(col=col, grid=grid, pos=pos, amp=amp, sigma=clutter_sigma(amp, clutter_db), squint=squint))
+    return out

@@ -163,6 +163,22 @@ int fsh_ffbp_set_profile(void* handle, int on);
 int fsh_ffbp_profile(void* handle, int* stage, int* level, float* ms, int cap, int* count);
 int fsh_ffbp_destroy(void* handle);
 
+/* ------------------------------------------------------------------------------------------ batches
+ * Many images in one launch: no reference counterpart (the reference forms patches one former call at a time,
+ * fastsar/patches.py:790-870). fsh_ffbp_enqueue puts one image's launches on a stream without waiting, allocating
+ * or copying: S [P, K] complex64 and image [nx, ny] complex64 are device buffers; the former must have formed once
+ * (its group buffers exist). Its NaN scan result is read afterwards with fsh_ffbp_bad_pulse (-1: none). One enqueue
+ * at a time per former handle until the stream has run it. */
+int fsh_ffbp_enqueue(void* handle, const void* S, void* image, void* stream);
+int fsh_ffbp_bad_pulse(void* handle, int* bad_pulse);
+/* n formers, each with its device history and image, captured into one hipGraph over nstreams forked streams
+ * (image i on stream i % nstreams), with a device wall-clock stamp at the start and after each image. */
+int fsh_batch_create(void* const* formers, const void* const* S, void* const* images, int n, int nstreams, void** batch);
+/* one launch of the graph, waited for: ms the whole batch (host clock), done_ms [n] (or NULL) each image's completion
+ * after the batch's start by the device wall clock; bad [n] (or NULL) each image's first non-finite pulse or -1 */
+int fsh_batch_run(void* batch, float* ms, float* done_ms, int* bad);
+int fsh_batch_destroy(void* batch);
+
 /* ------------------------------------------------------------------------------------------ exact former
  * ExactFormer's geometry as planned on the host (fastsar/exact.py:415-540): tiles of tx x ty pixels with centres
  * cen, profiles of nfft bins, of which W reachable ones are kept per pulse from bin blo[p] (zero delay at nfft / 2).

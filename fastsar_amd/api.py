@@ -253,10 +253,10 @@ class ImageFormer:
         fastsar_amd.precision.pack) -> complex64 image [nx, ny]. The decode, the window and the scan for NaN and inf
         run on the device. out: an optional DeviceArray for the image."""
         from . import _hip
-        from .precision import History16
-        if isinstance(S, History16):
+        from .precision import TypedHistory
+        if isinstance(S, TypedHistory):
             if self._gate is not None:
-                raise ValueError('a 16-bit history is formed ungated: pass it as complex64 to a gated former')
+                raise ValueError('a typed history is formed ungated: pass it as complex64 to a gated former')
             return self._form.form16(S, out=out)
         if not isinstance(S, _hip.DeviceArray):
             S = np.asarray(S)

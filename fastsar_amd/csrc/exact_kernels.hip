@@ -37,11 +37,22 @@ template <int DT>
 __device__ __forceinline__ float2 load_sample(const void* s, size_t idx, const float* inv, int p)
 {
     if (DT == 0) return static_cast<const float2*>(s)[idx];
+    if (DT == 4) {                                 // CPHD CI2: int8 pairs, times AmpSF
+        const unsigned short w = static_cast<const unsigned short*>(s)[idx];
+        const float g = inv ? inv[p] : 1.f;
+        return make_float2((float)(signed char)(w & 0xff) * g, (float)(signed char)(w >> 8) * g);
+    }
     const unsigned w = static_cast<const unsigned*>(s)[idx];
     if (DT == 1) {
         const half2_t h = __builtin_bit_cast(half2_t, w);
         const float g = inv[p];
         return make_float2((float)h.x * g, (float)h.y * g);
+    }
+    if (DT == 3) {                                 // CPHD CI4: big-endian int16 pairs, times AmpSF
+        const short re = (short)(((w & 0xffu) << 8) | ((w >> 8) & 0xffu));
+        const short im = (short)((((w >> 16) & 0xffu) << 8) | (w >> 24));
+        const float g = inv ? inv[p] : 1.f;
+        return make_float2((float)re * g, (float)im * g);
     }
     return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
 }
@@ -422,7 +433,7 @@ static int exact_form_impl(Exact* e, const void* S, int dtype, const float* puls
 {
     const fsh_exact_desc& d = e->d;
     int st;
-    const size_t ssz = dtype == FSH_C64 ? sizeof(float2) : 4;
+    const size_t ssz = dtype == FSH_C64 ? sizeof(float2) : dtype == FSH_CI8 ? 2 : 4;
     const char* src = static_cast<const char*>(S);
     const float* inv = pulse_scale;
     if (!S_on_device) {
@@ -457,6 +468,10 @@ static int exact_form_impl(Exact* e, const void* S, int dtype, const float* puls
             hipLaunchKernelGGL((k_pad_window<float2, 1>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv);
         else if (dtype == FSH_C32BF16)
             hipLaunchKernelGGL((k_pad_window<float2, 2>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv);
+        else if (dtype == FSH_CI16BE)
+            hipLaunchKernelGGL((k_pad_window<float2, 3>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv);
+        else if (dtype == FSH_CI8)
+            hipLaunchKernelGGL((k_pad_window<float2, 4>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv);
         else
             hipLaunchKernelGGL((k_pad_window<float2, 0>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv);
         FSH_TRY(hipGetLastError());
@@ -500,10 +515,7 @@ int fsh_exact_form_typed(void* handle, const void* S, int dtype, const float* pu
                          int image_on_device, int* bad_pulse)
 {
     if (!handle || !S || !image) { set_error("exact_form_typed: null argument"); return E_INVALID; }
-    if ((dtype != FSH_C64 && dtype != FSH_C32F16 && dtype != FSH_C32BF16) || ((pulse_scale != nullptr) != (dtype == FSH_C32F16))) {
-        set_error("exact_form_typed: a float16 history goes with per-pulse scales, the other types with none");
-        return E_INVALID;
-    }
+    if (check_history_type(dtype, pulse_scale)) return E_INVALID;
     return exact_form_impl(static_cast<Exact*>(handle), S, dtype, pulse_scale, S_on_device, image, image_on_device, bad_pulse);
 }
 

@@ -819,26 +819,44 @@ int launch_prepare(const float2* S, int P, int K, const float* wp, const float* 
     return OK;
 }
 
-// 16-bit histories: (Re, Im) as float16 (FSH_C32F16, each pulse stored times a power of two, undone by inv[p]) or
-// bfloat16 (FSH_C32BF16) in one 32-bit word per sample; decoded to float32, windowed and scanned like k_prepare
+// Histories in other types, decoded to float32, windowed and scanned like k_prepare: (Re, Im) as float16
+// (FSH_C32F16, each pulse stored times a power of two, undone by inv[p]) or bfloat16 (FSH_C32BF16) in one 32-bit
+// word per sample; or a CPHD signal array as it lies in the file (fastsar/io.py:128-143, 160-172): big-endian int16
+// pairs (FSH_CI16BE, CI4) or int8 pairs (FSH_CI8, CI2), converted to float32 and times the pulse's AmpSF (inv[p],
+// or 1 where inv is null)
 template <int DT>
-__global__ __launch_bounds__(256) void k_prepare16(const unsigned* __restrict__ S, int P, int K, const float* __restrict__ inv,
+__device__ __forceinline__ float2 decode_sample(const void* S, size_t i, const float* inv, int p)
+{
+    if (DT == 4) {
+        const unsigned short w = static_cast<const unsigned short*>(S)[i];
+        const float g = inv ? inv[p] : 1.f;
+        return make_float2((float)(signed char)(w & 0xff) * g, (float)(signed char)(w >> 8) * g);
+    }
+    const unsigned w = static_cast<const unsigned*>(S)[i];
+    if (DT == 1) {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        const h2 h = __builtin_bit_cast(h2, w);
+        const float g = inv ? inv[p] : 1.f;
+        return make_float2((float)h.x * g, (float)h.y * g);
+    }
+    if (DT == 3) {
+        const short re = (short)(((w & 0xffu) << 8) | ((w >> 8) & 0xffu));
+        const short im = (short)((((w >> 16) & 0xffu) << 8) | (w >> 24));
+        const float g = inv ? inv[p] : 1.f;
+        return make_float2((float)re * g, (float)im * g);
+    }
+    return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+}
+
+template <int DT>
+__global__ __launch_bounds__(256) void k_prepare16(const void* __restrict__ S, int P, int K, const float* __restrict__ inv,
                                                     const float* __restrict__ wp, const float* __restrict__ wk,
                                                     float2* __restrict__ x, int* __restrict__ bad)
 {
     const size_t n = (size_t)P * K;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const unsigned w = S[i];
         const int p = (int)(i / K);
-        float2 v;
-        if (DT == 1) {
-            typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-            const h2 h = __builtin_bit_cast(h2, w);
-            const float g = inv ? inv[p] : 1.f;
-            v = make_float2((float)h.x * g, (float)h.y * g);
-        } else {
-            v = make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
-        }
+        float2 v = decode_sample<DT>(S, i, inv, p);
         if (!(isfinite(v.x) && isfinite(v.y))) atomicMin(bad, p);
         if (wp) {
             const float g = wp[p] * wk[i - (size_t)p * K];
@@ -854,9 +872,13 @@ int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, c
     const size_t n = (size_t)P * K;
     const int blocks = (int)std::min<size_t>((n + 255) / 256, 1 << 16);
     if (dtype == 1)
-        hipLaunchKernelGGL(k_prepare16<1>, dim3(blocks), dim3(256), 0, s, (const unsigned*)S, P, K, inv, wp, wk, x, bad);
+        hipLaunchKernelGGL(k_prepare16<1>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad);
     else if (dtype == 2)
-        hipLaunchKernelGGL(k_prepare16<2>, dim3(blocks), dim3(256), 0, s, (const unsigned*)S, P, K, inv, wp, wk, x, bad);
+        hipLaunchKernelGGL(k_prepare16<2>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad);
+    else if (dtype == 3)
+        hipLaunchKernelGGL(k_prepare16<3>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad);
+    else if (dtype == 4)
+        hipLaunchKernelGGL(k_prepare16<4>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad);
     else {
         set_error("prepare: unknown history type");
         return E_INVALID;

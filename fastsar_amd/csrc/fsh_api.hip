@@ -319,6 +319,23 @@ struct Former {
     }
 };
 
+int check_history_type(int dtype, const void* pulse_scale)
+{
+    if (dtype < FSH_C64 || dtype > FSH_CI8) {
+        set_error("form_typed: unknown history type " + std::to_string(dtype));
+        return E_INVALID;
+    }
+    if (dtype == FSH_C32F16 && !pulse_scale) {
+        set_error("form_typed: a float16 history goes with its per-pulse scales");
+        return E_INVALID;
+    }
+    if ((dtype == FSH_C64 || dtype == FSH_C32BF16) && pulse_scale) {
+        set_error("form_typed: complex64 and bfloat16 histories take no per-pulse scales");
+        return E_INVALID;
+    }
+    return OK;
+}
+
 }  // namespace fsh
 
 using namespace fsh;
@@ -607,7 +624,7 @@ static int ffbp_form_impl(Former* f, const void* S, int dtype, const float* puls
     for (int i = 0; i < ST_N; ++i) f->stage_ms[i] = 0.f;
     f->recs.clear();
     f->cur_level = -1;
-    const size_t ssz = dtype == FSH_C64 ? sizeof(float2) : 4;
+    const size_t ssz = dtype == FSH_C64 ? sizeof(float2) : dtype == FSH_CI8 ? 2 : 4;
     const void* src = S;
     const float* inv = pulse_scale;
     if (!S_on_device) {
@@ -673,14 +690,7 @@ int fsh_ffbp_form_typed(void* handle, const void* S, int dtype, const float* pul
                         int image_on_device, int* bad_pulse)
 {
     if (!handle || !S || !image) { set_error("ffbp_form_typed: null argument"); return E_INVALID; }
-    if (dtype != FSH_C64 && dtype != FSH_C32F16 && dtype != FSH_C32BF16) {
-        set_error("ffbp_form_typed: unknown history type");
-        return E_INVALID;
-    }
-    if ((pulse_scale != nullptr) != (dtype == FSH_C32F16)) {
-        set_error("ffbp_form_typed: a float16 history goes with per-pulse scales, the other types with none");
-        return E_INVALID;
-    }
+    if (check_history_type(dtype, pulse_scale)) return E_INVALID;
     return ffbp_form_impl(static_cast<Former*>(handle), S, dtype, pulse_scale, S_on_device, image, image_on_device, bad_pulse);
 }
 

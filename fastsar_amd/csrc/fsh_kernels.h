@@ -6,6 +6,8 @@
 namespace fsh {
 
 int launch_prepare(const float2* S, int P, int K, const float* wp, const float* wk, float2* x, int* bad, hipStream_t s);
+// the history types of fsh_ffbp_form_typed and fsh_exact_form_typed with their scale rule: OK or E_INVALID (message set)
+int check_history_type(int dtype, const void* pulse_scale);
 int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, const float* wp, const float* wk, float2* x,
                      int* bad, hipStream_t s);
 int launch_fill_int(int* p, int n, int value, hipStream_t s);

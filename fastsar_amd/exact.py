@@ -249,15 +249,17 @@ class ExactFormer:
         fastsar_amd.precision.pack) -> complex64 image [nx, ny] (or into out, a DeviceArray). NaN and inf are found
         on the device and raise ValueError naming the first pulse."""
         from .api import _check_history
-        from .precision import History16
-        if isinstance(S, History16):
+        from .precision import TypedHistory
+        if isinstance(S, TypedHistory):
             if S.shape != (self.P, self.K):
                 raise ValueError(f'phase history must be {(self.P, self.K)}, got {S.shape}')
             img_dev = isinstance(out, _hip.DeviceArray)
             img = out if img_dev else np.empty((self.nx, self.ny), np.complex64)
             bad = ctypes.c_int(-1)
             h = self._handle()
-            _hip.check(_bind().fsh_exact_form_typed(h, _hip.ptr(S.data), S.code, _hip.ptr(S.inv), int(S.on_device),
+            data = S.data if S.on_device else np.ascontiguousarray(S.data)
+            inv = S.inv if (S.on_device or S.inv is None) else np.ascontiguousarray(S.inv, np.float32)
+            _hip.check(_bind().fsh_exact_form_typed(h, _hip.ptr(data), S.code, _hip.ptr(inv), int(S.on_device),
                                                     _hip.ptr(img), int(img_dev), ctypes.byref(bad)), 'exact_form')
             return img
         on_dev = isinstance(S, _hip.DeviceArray)

@@ -137,13 +137,14 @@ class HipFFBP:
         return dict(zip(STAGES, [float(v) for v in ms]))
 
     def form16(self, h, out=None):
-        """h: a History16 (fastsar_amd.precision), host or device, decoded on the device; out as in __call__."""
+        """h: a TypedHistory (fastsar_amd.precision: 16-bit floats, CPHD integer samples), host or device, decoded
+        on the device; out as in __call__."""
         if h.shape != (self.P, self.K):
             raise ValueError(f'phase history must be {(self.P, self.K)}, got {h.shape}')
         img_dev = isinstance(out, _hip.DeviceArray)
         img = out if img_dev else np.empty((self.nx, self.ny), np.complex64)
         bad = ctypes.c_int(-1)
-        data = h.data if h.on_device else np.ascontiguousarray(h.data, np.uint32)
+        data = h.data if h.on_device else np.ascontiguousarray(h.data)
         inv = None if h.inv is None else (h.inv if h.on_device else np.ascontiguousarray(h.inv, np.float32))
         _hip.check(_hip.lib().fsh_ffbp_form_typed(self.h, _hip.ptr(data), h.code, _hip.ptr(inv), int(h.on_device),
                                                   _hip.ptr(img), int(img_dev), ctypes.byref(bad)), 'ffbp_form')

@@ -42,33 +42,74 @@ def pulse_exponents(S):
     return np.clip(e, -126, 126)
 
 
-class History16:
-    """A [P, K] complex history in 16-bit pairs: data uint32 [P, K] (host ndarray or DeviceArray), dtype 'float16'
-    or 'bfloat16', inv float32 [P] (float16: the per-pulse inverse scales, on the same side as data) or None."""
+class TypedHistory:
+    """A [P, K] history in a stored type the formers decode on the device (fsh_ffbp_form_typed): data (host ndarray
+    or DeviceArray) whose first two dimensions are [P, K], code (the FSH_* type), inv float32 [P] (per-pulse factors,
+    on the same side as data) or None."""
+
+    def __init__(self, data, code, inv=None):
+        self.data, self.code, self.inv = data, int(code), inv
+        self.shape = tuple(data.shape[:2])
+        self.on_device = isinstance(data, _hip.DeviceArray)
+
+    def _like(self, data, inv):
+        return TypedHistory(data, self.code, inv)
+
+    def to_device(self):
+        if self.on_device:
+            return self
+        return self._like(_hip.DeviceArray.from_host(np.ascontiguousarray(self.data)),
+                          None if self.inv is None else _hip.DeviceArray.from_host(np.ascontiguousarray(self.inv, np.float32)))
+
+    def to_host(self):
+        if not self.on_device:
+            return self
+        return self._like(self.data.to_host(), None if self.inv is None else self.inv.to_host())
+
+
+class History16(TypedHistory):
+    """A [P, K] complex history in 16-bit pairs: data uint32 [P, K], dtype 'float16' or 'bfloat16', inv float32 [P]
+    (float16: the per-pulse inverse scales) or None."""
 
     def __init__(self, data, dtype, inv=None):
         if dtype not in DTYPES:
             raise ValueError(f"dtype must be 'float16' or 'bfloat16', got {dtype!r}")
         if (inv is None) != (dtype == 'bfloat16'):
             raise ValueError('a float16 history carries per-pulse scales; a bfloat16 one carries none')
-        self.data, self.dtype, self.inv = data, dtype, inv
-        self.shape = tuple(data.shape)
-        self.on_device = isinstance(data, _hip.DeviceArray)
+        super().__init__(data, DTYPES[dtype], inv)
+        self.dtype = dtype
 
-    @property
-    def code(self):
-        return DTYPES[self.dtype]
+    def _like(self, data, inv):
+        return History16(data, self.dtype, inv)
 
-    def to_device(self):
-        if self.on_device:
-            return self
-        return History16(_hip.DeviceArray.from_host(self.data), self.dtype,
-                         None if self.inv is None else _hip.DeviceArray.from_host(self.inv))
 
-    def to_host(self):
-        if not self.on_device:
-            return self
-        return History16(self.data.to_host(), self.dtype, None if self.inv is None else self.inv.to_host())
+class CphdSignal(TypedHistory):
+    """A CPHD signal array as it lies in the file, for the formers to convert on the device (the conversion of
+    fastsar/io.py:128-172): samples [P, K, 2] big-endian int16 (CI4, '>i2') or int8 (CI2), and the vectors' AmpSF
+    [P] (or None for 1). The decoded sample is float32(Re, Im) times AmpSF in float32."""
+
+    def __init__(self, samples, amp=None):
+        if isinstance(samples, _hip.DeviceArray):
+            raise ValueError('CphdSignal takes the host array; call to_device() on it')
+        a = np.asarray(samples)
+        if a.ndim != 3 or a.shape[2] != 2:
+            raise ValueError(f'CPHD samples must be [pulses, samples, 2], got shape {a.shape}')
+        if a.dtype == np.dtype('>i2'):
+            code = 3
+        elif a.dtype == np.int8:
+            code = 4
+        else:
+            raise ValueError(f"CPHD samples must be big-endian int16 ('>i2', CI4) or int8 (CI2), got {a.dtype}")
+        inv = None
+        if amp is not None:
+            inv = np.ascontiguousarray(amp, np.float32)
+            if inv.shape != (a.shape[0],):
+                raise ValueError(f'AmpSF must hold one factor per pulse ({a.shape[0]}), got shape {inv.shape}')
+        super().__init__(np.ascontiguousarray(a), code, inv)
+
+    def _like(self, data, inv):
+        t = TypedHistory(data, self.code, inv)
+        return t
 
 
 def pack(S, dtype='float16'):

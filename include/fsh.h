@@ -141,13 +141,20 @@ int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, voi
  * non-finite sample when FSH_ENONFINITE is returned. */
 int fsh_ffbp_form(void* handle, const void* S, int S_on_device, void* image, int image_on_device, int* bad_pulse);
 /* First-level children per group (0: automatic, up to 8, halved while device memory lacks). */
-/* The same with a history in a given type: FSH_C64 complex64 (as fsh_ffbp_form); FSH_C32F16 (Re, Im) float16 pairs,
- * pulse p stored times a power of two that pulse_scale[p] undoes (its inverse, [P] float32, host or device as S);
- * FSH_C32BF16 (Re, Im) bfloat16 pairs (pulse_scale NULL). Decoded to float32 on the device before the window. No
- * reference counterpart: the 16-bit storage of the precision variants (half the upload and history footprint). */
+/* The same with a history in a given type, decoded to float32 on the device before the window:
+ *   FSH_C64      complex64 (as fsh_ffbp_form), pulse_scale NULL;
+ *   FSH_C32F16   (Re, Im) float16 pairs, pulse p stored times a power of two that pulse_scale[p] undoes;
+ *   FSH_C32BF16  (Re, Im) bfloat16 pairs, pulse_scale NULL;
+ *   FSH_CI16BE   a CPHD CI4 signal array as it lies in the file: big-endian int16 (Re, Im) pairs, times
+ *                pulse_scale[p] (the vector's AmpSF) or 1 where NULL (fastsar/io.py:128-172 converts on the host);
+ *   FSH_CI8      a CPHD CI2 array: int8 (Re, Im) pairs, times AmpSF likewise.
+ * pulse_scale is [P] float32, host or device as S. The 16-bit float types have no reference counterpart (the storage
+ * of the precision variants: half the upload and history footprint). */
 #define FSH_C64 0
 #define FSH_C32F16 1
 #define FSH_C32BF16 2
+#define FSH_CI16BE 3
+#define FSH_CI8 4
 int fsh_ffbp_form_typed(void* handle, const void* S, int dtype, const float* pulse_scale, int S_on_device, void* image,
                         int image_on_device, int* bad_pulse);
 int fsh_ffbp_set_group(void* handle, int ng);

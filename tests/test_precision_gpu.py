@@ -65,3 +65,29 @@ def test_precision_table(c1):
             bound = 10 * np.log10(10 ** (rows[(algo, 'complex64')] / 10) + 2 * store[dt])
             assert rows[(algo, dt)] < bound, (algo, dt, rows[(algo, dt)], bound)
     assert rows[('ffbp', 'complex64')] < -60 and rows[('bp', 'complex64')] < -65
+
+
+@pytest.mark.parametrize('kind', ['CI4', 'CI2'])
+@pytest.mark.parametrize('amp', [False, True])
+@pytest.mark.parametrize('algo', ['ffbp', 'bp'])
+def test_cphd_integer_samples(c1, kind, amp, algo):
+    """CPHD integer signal arrays converted on the device: the image equals, bit for bit, the image of the history
+    the reference's reader would build on the host (float32(Re, Im) times AmpSF in float32, fastsar/io.py:160-172)"""
+    col, S, grid = c1
+    rng = np.random.default_rng(4)
+    lim = 32000 if kind == 'CI4' else 120
+    q = np.abs(np.stack([S.real, S.imag], -1)).max()
+    ints = np.rint(np.stack([S.real, S.imag], -1) / q * lim)
+    raw = ints.astype('>i2') if kind == 'CI4' else ints.astype(np.int8)
+    a = (rng.uniform(0.5, 2.0, col.Np).astype(np.float32)) if amp else None
+    host = np.empty(S.shape, np.complex64)
+    host.real, host.imag = raw[..., 0], raw[..., 1]
+    if a is not None:
+        host *= a[:, None]
+    f = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid) if algo == 'ffbp' else \
+        ExactFormer(col.ant, col.fmin, col.df, col.K, **grid)
+    print(f'cphd {kind} amp={amp} {algo}', flush=True)
+    want = f(host)
+    sig = pr.CphdSignal(raw, a)
+    assert np.array_equal(f(sig), want)
+    assert np.array_equal(f(sig.to_device()), want)

@@ -196,6 +196,63 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_learned(args):
+    """The learned decimation filters on the small spotlight geometry at 1024 pulses (512^2 px): image error against
+    the FP64 checker on held-out scenes and time per image, for the plan's Kaiser filters (70 and 80 dB), the learned
+    filters of 6 and 8 taps per phase (the committed ones, trained on 200 scenes) and the same-length Kaiser
+    fallback they correct."""
+    from fastsar_amd import ImageFormer, _hip, learned
+    from fastsar_amd.exact import bp_points_f64_device
+    from fastsar_amd.metrics import rel_db
+    from fastsar_amd.api import _window
+    col, grid, plan = learned.c1_geometry()
+    wp, wk = _window(col.Np, col.K)
+    pts = np.stack(np.meshgrid(np.arange(grid['nx']), np.arange(grid['ny']), indexing='ij'), -1).reshape(-1, 2)
+    e1, e2 = np.asarray(grid['e1']), np.asarray(grid['e2'])
+    xyz = ((pts[:, 0] - grid['nx'] / 2.0) * grid['spx'])[:, None] * e1 + ((pts[:, 1] - grid['ny'] / 2.0) * grid['spy'])[:, None] * e2
+    args_f = (col.ant, col.fmin, col.df, col.K)
+    kw = dict(T=16, levels=3)
+    forms = dict(kaiser70=ImageFormer(*args_f, **grid, **kw), kaiser80=ImageFormer(*args_f, **grid, **kw, atten=80.0))
+    taps_of = {}
+    for T in (6, 8):
+        t = learned.default_taps(T)
+        if t is None:
+            log(f'no committed filters of {T} taps per phase; training on 200 scenes')
+            t = learned.train_c1(T, 200, log=log)
+        taps_of[T] = t
+        forms[f'learned{T}'] = ImageFormer(*args_f, **grid, **kw, taps=t)
+        forms[f'fallback{T}'] = ImageFormer(*args_f, **grid, **kw, taps=learned.LearnedTaps(T, 3, {}))
+    res = {k: dict(err=[], ms=[]) for k in forms}
+    img = _hip.DeviceArray((grid['nx'], grid['ny']), np.complex64)
+    for seed in range(5000, 5000 + args.learned_scenes):
+        dS = learned.c1_scene(col, seed)
+        nfft = 1 << int(np.ceil(np.log2(64 * col.K)))
+        ref = bp_points_f64_device(dS, col.ant, np.linalg.norm(col.ant, axis=1), col.fmin, col.df, xyz, nfft, True, wp, wk,
+                                   chunk=64)
+        for k, f in forms.items():
+            f(dS, out=img)
+            _hip.check(_hip.lib().fsh_synchronize())
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                f(dS, out=img)
+            _hip.check(_hip.lib().fsh_synchronize())
+            res[k]['ms'].append((time.perf_counter() - t0) / args.steps * 1e3)
+            res[k]['err'].append(rel_db(img.to_host().ravel(), ref))
+    rows = {}
+    for k, f in forms.items():
+        lv = f._plan['levels']
+        rows[k] = dict(error_db=round(float(np.mean(res[k]['err'])), 2), ms=round(float(np.median(res[k]['ms'])), 3),
+                       range_taps=[l['fir_k']['L'] if l['fir_k'] else 1 for l in lv],
+                       pulse_taps=[l['fir_p']['L'] if l['fir_p'] else 1 for l in lv])
+        log(f'{k:10s} {rows[k]}')
+    out = dict(metric='learned decimators: image error (dB vs FP64) and ms per 512^2 image from 1024 x 1024', value=rows['learned6']['error_db'],
+               unit='dB', n_gpus=1, steps=args.steps, warmup=1, ms_per_step=rows['learned6']['ms'], higher_is_better=False,
+               scaling='weak', vs_baseline=None, dtype='f32', data='synthetic',
+               config=dict(workload=f'512x512 from 1024 pulses x 1024 samples, T=16, 3 levels, {args.learned_scenes} held-out scenes',
+                           forms=rows, training={T: t.history for T, t in taps_of.items()}))
+    print(json.dumps(out, default=float), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
@@ -212,6 +269,8 @@ def main():
     ap.add_argument('--no-profile', action='store_true', help='timed steps only (for counter runs under rocprofv3)')
     ap.add_argument('--calibrate-copy', action='store_true', help='also run the plain copy kernel on 1 GiB three times')
     ap.add_argument('--small', action='store_true', help='the 512 x 512 scene instead (quick check)')
+    ap.add_argument('--learned', action='store_true', help='the learned decimation filters against the Kaiser ones')
+    ap.add_argument('--learned-scenes', type=int, default=4)
     ap.add_argument('--batch', action='store_true', help='the batched workload of uneven images instead')
     ap.add_argument('--batch-images', type=int, default=48)
     ap.add_argument('--batch-streams', type=int, default=4)
@@ -226,6 +285,8 @@ def main():
 
     if args.batch:
         return bench_batch(args)
+    if args.learned:
+        return bench_learned(args)
     from fastsar_amd import ImageFormer, _hip, sim
     from fastsar_amd.exact import simulate_device, bp_points_f64_device
     from fastsar_amd.metrics import rel_db, stage_bytes, stage_flops

@@ -159,11 +159,12 @@ class ImageFormer:
     not gate, since the hip first level holds any range filter.
     atten: stopband attenuation (dB) of the levels' Kaiser decimation filters; 70 (the reference's design, whose
     plans this reproduces bit for bit) sets an image error floor near -60 dB, 80 lowers it by about 10 dB for
-    filters about 16 percent longer."""
+    filters about 16 percent longer. taps: learned decimation filters (fastsar_amd.learned.LearnedTaps, trained for
+    the plan's number of levels) in place of the Kaiser ones."""
 
     def __init__(self, ant, fmin, df, K, nx, ny, spx, spy, e1=(1.0, 0.0, 0.0), e2=(0.0, 1.0, 0.0), *, backend='auto',
                  precision='float32', window=True, T='auto', levels=None, pmax=0.4, target_db=-40.0, aperture_weight=None,
-                 ref=None, range_gate='auto', padded=None, atten=70.0):
+                 ref=None, range_gate='auto', padded=None, atten=70.0, taps=None):
         if not (np.isfinite(atten) and 40.0 <= float(atten) <= 140.0):
             raise ValueError(f'atten must be a stopband attenuation between 40 and 140 dB, got {atten!r}')
         self.ant = _check_positions(ant)
@@ -227,6 +228,9 @@ class ImageFormer:
                 self.range_gate = dict(g, K=self.K, kept=kept, note=(
                     f'range gate: each pulse cut to the {g["N"]} range bins about the image before formation, '
                     f'{kept} samples for {self.K}'))
+        if taps is not None:
+            # learned decimation filters (fastsar_amd.learned.LearnedTaps) in place of the plan's Kaiser ones
+            plan = taps.apply(plan)
         self._Kf = plan['levels'][0]['K']
         self.levels = nlev
         self._plan = plan

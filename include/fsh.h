@@ -238,6 +238,30 @@ int fsh_bp_points_f64(const float* S, int P, int K, const float* wp, const float
                       const double* ref, const double* pts, long N, double fmin, double df, int nfft, int cubic,
                       double* out, int chunk);
 
+/* ------------------------------------------------------------------------------------------ learned filters
+ * Training of decimation filters that replace the levels' Kaiser decimators (fastsar/ffbp2.py:103-137, 240-242 make
+ * them; no learned counterpart exists in the reference). All pointers are device pointers.
+ * fsh_ls_stats: over the outputs j0 <= j < j0 + n_out of every line of a decimation stage, the window x_j =
+ * input[D j + r - pl], r < L (zero outside [0, n_in)), and the exact output y_j, R += sum Re(x x^H) [L x L],
+ * v += sum Re(x conj y) [L], E += sum |y|^2 in float64 (complex64 data). Line q * inner + w starts at
+ * x + q xq + w xr with step xs (in complex samples), and at y + q yq + w yr with step ys. L <= 128. */
+int fsh_ls_stats(const float* x, const float* y, long nlines, int inner, long xq, long xr, long xs, long yq, long yr, long ys,
+                 int n_in, int j0, int n_out, int D, int pl, int L, double* R, double* v, double* E, void* stream);
+/* The tap network: an MLP [f, s] -> tanh 64 -> tanh 64 -> T taps, float32 parameters (W1 [64 x 2], b1, W2 [64 x 64],
+ * b2, W3 [T x 64], b3), fsh_tapnet_nparam(T) of them. forward: in [n, 2], act [n, 128] (kept for backward),
+ * out [n, T]. backward: dout [n, T] -> grad [nparam] (grow [n, nparam] scratch). ls_grad: per stage s, taps
+ * h[D u + ph] = base[128 s + D u + ph] (or 0 where base is NULL) + out[(off[s] + ph) T + u] for D u + ph < L[s],
+ * loss[s] = (h^T R h - 2 h^T v + E) / E with R at R + R_at[s] (row-major L x L), v at v + 128 s, E[s]; dout of the
+ * stage's points = wt[s] d loss / d h. adam: one Adam step t >= 1. */
+int fsh_tapnet_nparam(int T);
+int fsh_tapnet_forward(const float* prm, int T, const float* in, int n, float* act, float* out, void* stream);
+int fsh_tapnet_backward(const float* prm, int T, const float* in, int n, const float* act, const float* dout, float* grow,
+                        float* grad, void* stream);
+int fsh_tapnet_ls_grad(const float* out, int T, int nstage, const int* st_D, const int* st_L, const int* st_off,
+                       const long* R_at, const double* R, const double* v, const double* E, const double* base,
+                       const float* wt, float* dout, double* loss, void* stream);
+int fsh_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps, int t, void* stream);
+
 /* ------------------------------------------------------------------------------------------ synthetic data
  * Test and benchmark scenes (no reference counterpart on the device; the host sum is fastsar/sim.py:90-105). */
 /* S [P, K] complex += sum_n amp_n exp(-j 4 pi f_k (|pos_n - ant_p| - ref_p) / c), f_k = fmin + k df; all device. */

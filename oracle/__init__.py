@@ -7,6 +7,8 @@ Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this
       bp_points  FP64 exact backprojection from float64 profiles (fastsar/bp.py:55-63, 99-138; tests/test_bp.py:23-35)
       exact_tiles  ExactFormer's tiled algorithm in float64 (fastsar/exact.py:240-328)
       range_gate   the range gate of a wide-band history in complex128 (fastsar/api.py:546-563)
+      tapnet_*, ls_loss_grad  the learned decimators' network and least-squares loss in float64 (no reference
+                   counterpart: the checker of the HIP training kernels)
   * the reference's own CPU kernels, compiled in place from its sources into oracle/_ref/ (ref_* below);
   * the orchestration of the CPU former restated in Python (fastsar/ffbp_cpu.py:158-291), which drives either
     set of stages: ffbp_image(..., stages='ref') is the reference's CPU path, stages='oracle' the float64 one;
@@ -23,6 +25,8 @@ How each function is pinned (tests/test_oracle.py):
   ffbp_image                      stages='ref' against stages='oracle' (-100 dB), and against bp_points within the
                                   final-stage model's predicted error (tests/test_accuracy.py:29-54)
   window                          scipy.signal.windows.taylor, the library routine the reference calls
+  tapnet_forward/backward,        central finite differences of the float64 forward pass and loss (relative 1e-6;
+  ls_loss_grad                    tests/test_learned.py), and the loss against the direct sum over the pairs
   range_gate                      the image it forms: float64 exact backprojection of the gated and the whole history
                                   agree to -98 dB (no fitted gain), and planted mistakes in the range cut, the
                                   transform directions or the Kp / N scale each fail that check
@@ -448,3 +452,61 @@ def exact_tiles(win, lo, ant, ref, cen, ntx, nty, tx, ty, nx, ny, spx, spy, e1, 
                           nx, ny, spx, spy, np.asarray(e1, np.float64), np.asarray(e2, np.float64), inv_dr, kcyc,
                           int(cubic), ore, oim)
     return (ore + 1j * oim).reshape(nx, ny)
+
+
+# ------------------------------------------------------------------------------------------------ tap network
+
+HID = 64
+
+
+def _tapnet_split(params, T):
+    p = np.asarray(params, np.float64)
+    o = [0]
+
+    def take(n, shape):
+        a = p[o[0]:o[0] + n].reshape(shape)
+        o[0] += n
+        return a
+    W1, b1 = take(2 * HID, (HID, 2)), take(HID, (HID,))
+    W2, b2 = take(HID * HID, (HID, HID)), take(HID, (HID,))
+    W3, b3 = take(T * HID, (T, HID)), take(T, (T,))
+    return W1, b1, W2, b2, W3, b3
+
+
+def tapnet_forward(params, T, pts):
+    """The tap network in float64: pts [n, 2] -> (out [n, T], h1, h2)."""
+    W1, b1, W2, b2, W3, b3 = _tapnet_split(params, T)
+    h1 = np.tanh(np.asarray(pts, np.float64) @ W1.T + b1)
+    h2 = np.tanh(h1 @ W2.T + b2)
+    return h2 @ W3.T + b3, h1, h2
+
+
+def tapnet_backward(params, T, pts, dout):
+    """d loss / d params [nparam] (float64, the device's parameter order) from d loss / d out [n, T], by the chain
+    rule through tanh (1 - h^2)."""
+    W1, b1, W2, b2, W3, b3 = _tapnet_split(params, T)
+    x = np.asarray(pts, np.float64)
+    _, h1, h2 = tapnet_forward(params, T, pts)
+    g = np.asarray(dout, np.float64)
+    gW3, gb3 = g.T @ h2, g.sum(0)
+    d2 = (g @ W3) * (1 - h2 ** 2)
+    gW2, gb2 = d2.T @ h1, d2.sum(0)
+    d1 = (d2 @ W2) * (1 - h1 ** 2)
+    gW1, gb1 = d1.T @ x, d1.sum(0)
+    return np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])
+
+
+def ls_loss_grad(out, T, stages, wt=None):
+    """The least-squares loss of filters built from network outputs: stages [(D, L, off, R [L, L], v [L], E,
+    base [L])]; taps h[D u + ph] = base + out[off + ph, u]; loss_s = (h R h - 2 h v + E) / E; returns (loss [S],
+    d (sum wt_s loss_s) / d out)."""
+    out = np.asarray(out, np.float64)
+    dout = np.zeros_like(out)
+    loss = []
+    wt = np.ones(len(stages)) if wt is None else np.asarray(wt, np.float64)
+    for s, (D, L, off, R, v, E, base) in enumerate(stages):
+        r = np.arange(L)
+        h = np.asarray(base, np.float64)[:L] + out[off + r % D, r // D]
+        loss.append((h @ R @ h - 2 * h @ v + E) / E)
+        dout[off + r % D, r // D] += wt[s] * 2 * (R @ h - v) / E
+    return np.array(loss), dout

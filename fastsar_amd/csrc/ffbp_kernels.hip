@@ -709,11 +709,15 @@ __global__ __launch_bounds__(256) void k_final_tiles_reg(const float2* __restric
                                                           double a1, double fc2, const float* __restrict__ w0,
                                                           const float* __restrict__ gx, const float* __restrict__ gyr,
                                                           const int* __restrict__ tile_xy, int ox, int oy, int nx, int ny,
-                                                          float2* __restrict__ out)
+                                                          float2* __restrict__ out, int SP)
 {
     constexpr int BI = T < 8 ? T : 8;
     constexpr int NB = T / BI;                     // blocks per tile side
-    const long gid = (long)blockIdx.x * 256 + threadIdx.x;
+    // SP consecutive lanes (a power of two up to 64) share one pixel block, each summing the subapertures
+    // p = sp, sp + SP, ...; the partial sums are added across those lanes in a fixed butterfly order
+    const long gid0 = (long)blockIdx.x * 256 + threadIdx.x;
+    const long gid = gid0 / SP;
+    const int sp = (int)(gid0 - gid * SP);
     const long b = gid / (NB * NB);
     if (b >= B) return;
     const int blk = (int)(gid - b * NB * NB), bx = blk / NB, by = blk - bx * NB;
@@ -726,7 +730,7 @@ __global__ __launch_bounds__(256) void k_final_tiles_reg(const float2* __restric
         #pragma unroll
         for (int c = 0; c < BI; ++c) acc[a][c] = make_float2(0.f, 0.f);
     const float2* dp = data + (size_t)b * Pf * Qf;
-    for (int p = 0; p < Pf; ++p) {
+    for (int p = sp; p < Pf; p += SP) {
         const double gxp = ux[(size_t)b * Pf + p], gyp = uy[(size_t)b * Pf + p];
         float2 ra, rb, ba, bb, sra, srb, sba, sbb;
         cis_cyc(-(gxp * spx) * a0, ra.x, ra.y);         // ratio along i at q = 0
@@ -778,6 +782,16 @@ __global__ __launch_bounds__(256) void k_final_tiles_reg(const float2* __restric
             bb = cmul(bb, sbb);
         }
     }
+    for (int m = 1; m < SP; m <<= 1) {
+        #pragma unroll
+        for (int a = 0; a < BI; ++a)
+            #pragma unroll
+            for (int c = 0; c < BI; ++c) {
+                acc[a][c].x += __shfl_xor(acc[a][c].x, m, 64);
+                acc[a][c].y += __shfl_xor(acc[a][c].y, m, 64);
+            }
+    }
+    if (sp != 0) return;
     const double cx = tgeo[3 * b], cy = tgeo[3 * b + 1], f = fc2 / (2.0 * tgeo[3 * b + 2]);
     const int X0 = tile_xy[2 * b] - ox + i0, Y0 = tile_xy[2 * b + 1] - oy + j0;
     #pragma unroll
@@ -1025,11 +1039,23 @@ int launch_final_geom(const double* fu, const double* fr0, const double* cen, in
     return OK;
 }
 
+int final_split(long tiles, int T)
+{
+    const long per = (long)(T / std::min(T, 8)) * (T / std::min(T, 8));
+    int sp = 1;
+    while (sp < 16 && tiles * per * sp < 65536) sp *= 2;
+    return sp;
+}
+
 int launch_final_tiles(const float2* data, int B, int Pf, int Qf, int T, const double* ux, const double* uy,
                        const double* tgeo, double spx, double spy, double a0, double a1, double fc2, const float* w0,
                        const float* gx, const float* gyr, const int* tile_xy, int ox, int oy, int nx, int ny, float2* out,
-                       hipStream_t s)
+                       hipStream_t s, int split)
 {
+    if (split < 1 || split > 64 || (split & (split - 1))) {
+        set_error("final_tiles: the subaperture split must be a power of two up to 64");
+        return E_INVALID;
+    }
     if (!(T == 4 || T == 8 || T == 16 || T == 32)) {
         set_error("final_tiles: T must be 4, 8, 16 or 32");
         return E_INVALID;
@@ -1038,10 +1064,10 @@ int launch_final_tiles(const float2* data, int B, int Pf, int Qf, int T, const d
     const char* force = getenv("FSH_FINAL_KERNEL");
     if (!(force && strcmp(force, "lds") == 0)) {
         const int per = (T / std::min(T, 8)) * (T / std::min(T, 8));
-        const long blocks = ((long)B * per + 255) / 256;
+        const long blocks = ((long)B * per * split + 255) / 256;
         const bool w = w0 != nullptr;
 #define FSH_FR(TT, WW) hipLaunchKernelGGL((k_final_tiles_reg<TT, WW>), dim3((unsigned)blocks), dim3(256), 0, s, data, B, Pf, Qf, ux, uy, \
-                                          tgeo, spx, spy, a0, a1, fc2, w0, gx, gyr, tile_xy, ox, oy, nx, ny, out)
+                                          tgeo, spx, spy, a0, a1, fc2, w0, gx, gyr, tile_xy, ox, oy, nx, ny, out, split)
         switch (T) {
             case 4: if (w) FSH_FR(4, true); else FSH_FR(4, false); break;
             case 8: if (w) FSH_FR(8, true); else FSH_FR(8, false); break;

@@ -194,6 +194,25 @@ struct Former {
         tiles = (size_t)parents;
     }
 
+    // device bytes of the buffers of a group of g first-level children
+    size_t group_bytes(int g) const
+    {
+        size_t mb, mp, tiles;
+        group_sizes(g, mb, mp, tiles);
+        return 2 * mb * sizeof(float2) + 2 * mp * sizeof(float) + tiles * (2 * p.Pf + 3) * sizeof(double);
+    }
+
+    // up to 8 first-level children per group; more, up to all of them, while their buffers stay within 4 GB (small
+    // images: fewer and larger launches, which fill the chip)
+    int default_group() const
+    {
+        const int G = lv[0].h.C;
+        int g = std::min(8, G);
+        const size_t budget = std::max(group_bytes(g), (size_t)4 << 30);
+        while (g < G && group_bytes(std::min(G, 2 * g)) <= budget) g = std::min(G, 2 * g);
+        return g;
+    }
+
     int alloc_group(int g)
     {
         release_group();
@@ -312,7 +331,8 @@ struct Former {
         mark_begin();
         if ((st = launch_final_tiles(cur, (int)tiles, p.Pf, p.Qf, p.T, ux, uy, tgeo, p.spx, p.spy, p.a0, p.a1, p.fc2,
                                      w0 ? w0 + t0 * p.Pf : nullptr, gx ? gx + t0 * p.Pf : nullptr,
-                                     gyr ? gyr + t0 * p.Pf : nullptr, tile_xy + 2 * t0, p.ox, p.oy, p.nx, p.ny, image, stream)))
+                                     gyr ? gyr + t0 * p.Pf : nullptr, tile_xy + 2 * t0, p.ox, p.oy, p.nx, p.ny, image, stream,
+                                     final_split(p.ntiles, p.T))))
             return st;
         mark_end(ST_FINAL);
         return OK;
@@ -468,7 +488,7 @@ int fsh_final_tiles(const float* d, int B, int Pf, int Qf, int T, const double* 
                     const float* gyr, const int* tile_xy, int ox, int oy, int nx, int ny, float* out, void* stream)
 {
     return launch_final_tiles((const float2*)d, B, Pf, Qf, T, ux, uy, tgeo, spx, spy, a0, a1, fc2, w0, gx, gyr, tile_xy, ox,
-                              oy, nx, ny, (float2*)out, (hipStream_t)stream);
+                              oy, nx, ny, (float2*)out, (hipStream_t)stream, final_split(B, T));
 }
 
 int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, void** handle)
@@ -655,9 +675,9 @@ static int ffbp_form_impl(Former* f, const void* S, int dtype, const float* puls
         set_error(buf);
         return E_NONFINITE;
     }
-    // the group: requested, else up to 8, halved while the buffers do not fit
+    // the group: requested, else the default (default_group), halved while the buffers do not fit
     const int G = f->lv[0].h.C;
-    int want = f->ng_req > 0 ? std::min(f->ng_req, G) : std::min(8, G);
+    int want = f->ng_req > 0 ? std::min(f->ng_req, G) : f->default_group();
     if (f->ng != want) {
         while (true) {
             st = f->alloc_group(want);

@@ -44,7 +44,10 @@ int launch_final_geom(const double* fu, const double* fr0, const double* cen, in
 int launch_final_tiles(const float2* data, int B, int Pf, int Qf, int T, const double* ux, const double* uy,
                        const double* tgeo, double spx, double spy, double a0, double a1, double fc2, const float* w0,
                        const float* gx, const float* gyr, const int* tile_xy, int ox, int oy, int nx, int ny, float2* out,
-                       hipStream_t s);
+                       hipStream_t s, int split);
+// subapertures summed by separate lanes per pixel block in the final stage: more threads where the image has few
+// tiles (a power of two, 1 for 65536 pixel blocks and more)
+int final_split(long tiles, int T);
 
 // device allocation through one helper, so that a test can make the n-th allocation fail (FSH_FAIL_ALLOC=n)
 int dev_alloc(void** p, size_t bytes);

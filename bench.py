@@ -121,7 +121,7 @@ def measured_traffic(label, former, args):
     kernel's reading), when the summary covers this workload; else None."""
     path = os.path.join(ROOT, 'profiles', f'ffbp_12288x8192_T{former.T}_hbm_traffic.md')
     if (args.small or former.levels != 4 or args.atten != 80.0 or args.taps or args.train_taps or
-            args.history != 'complex64' or not os.path.exists(path)):
+            args.history != 'complex64' or args.precision != 'float32' or not os.path.exists(path)):
         return None
     for line in open(path):
         if line.startswith('traffic_bytes_per_launch '):
@@ -285,6 +285,8 @@ def main():
     ap.add_argument('--taps', default=None, help='learned decimation filters (an .npz of fastsar_amd.learned) for the image')
     ap.add_argument('--train-taps', type=int, default=0, help='train filters of this many taps per phase on 2 scenes of '
                     'the workload first (seeds 11, 12), saved to gpurun_out/learned_T<n>.npz, and form with them')
+    ap.add_argument('--precision', default='float32', choices=['float32', 'float16'],
+                    help='float16: the two-pass levels store their intermediate rows in float16')
     ap.add_argument('--history', default='complex64', choices=['complex64', 'float16', 'bfloat16'],
                     help='storage of the phase history the former reads (float16: per-pulse power-of-two scaled)')
     args = ap.parse_args()
@@ -339,7 +341,7 @@ def main():
         taps = learned.LearnedTaps.load(args.taps)
     t0 = time.perf_counter()
     former = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T, levels=args.levels or None,
-                         atten=args.atten, taps=taps)
+                         atten=args.atten, taps=taps, precision=args.precision)
     plan = former._plan
     log(f'former built in {time.perf_counter() - t0:.2f} s: T={former.T}, levels={former.levels}, '
         f'splits={plan["split"]}, final {plan["final"]["P"]} x {plan["final"]["K"]}, predicted {former.predicted_error_db:.1f} dB')
@@ -441,7 +443,7 @@ def main():
     learned_form = None
     lpath = os.path.join(ROOT, 'fastsar_amd', 'data', f'learned_c2_T8.npz')
     if (not args.small and taps is None and args.T == 8 and args.levels == 4 and args.atten == 80.0
-            and args.history == 'complex64' and os.path.exists(lpath)):
+            and args.history == 'complex64' and args.precision == 'float32' and os.path.exists(lpath)):
         from fastsar_amd import learned
         lt = learned.LearnedTaps.load(lpath)
         lf = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T, levels=args.levels, atten=args.atten, taps=lt)
@@ -482,7 +484,7 @@ def main():
                config=dict(workload=(f'{"512x512 from 512x1024" if args.small else "12288x8192 image from 16384 pulses x 8192 samples"}'
                                      f' {args.history}, factorized backprojection, T={former.T}, {former.levels} levels, '
                                      + (f'learned filters of {taps.T} taps per phase' if taps else f'{args.atten:g} dB filters')),
-                           history=args.history,
+                           history=args.history, precision=args.precision,
                            s_per_image=round(dt, 4), error_db=round(err, 2), error_tiles=f'{len(tiles)} x {side}^2',
                            predicted_error_db=round(former.predicted_error_db, 2), parallelism='single GPU',
                            host_to_host_ms=None if h2h is None else round(h2h * 1e3, 1), learned_form=learned_form),

@@ -154,7 +154,8 @@ class ImageFormer:
     T: 'auto' (the largest of 32 and 16 whose predicted error meets target_db) or 4, 8, 16, 32. levels: None
     (three, more where three cannot split an axis) or a count. aperture_weight(points, pulses) -> [n, m]: per-pixel
     pulse weights applied in the final stage. ref [P]: each pulse's reference range when it is not |ant|.
-    precision: 'float32'. range_gate: True cuts each pulse to the range bins the image spans before formation (a
+    precision: 'float32', or 'float16' (the two-pass levels' intermediate rows stored in float16). range_gate: True
+    cuts each pulse to the range bins the image spans before formation (a
     wide-band history for a narrow image: fewer samples, a first level that decimates less); 'auto' and False do
     not gate, since the hip first level holds any range filter.
     atten: stopband attenuation (dB) of the levels' Kaiser decimation filters; 70 (the reference's design, whose
@@ -177,8 +178,8 @@ class ImageFormer:
         col = Collect(fmin=float(fmin), df=float(df), K=self.K, ant=self.ant, res=0.5)
         nx, ny, e1, e2 = _check_grid(nx, ny, spx, spy, e1, e2)
         self.backend = _backend(backend)
-        if precision not in ('float32',):
-            raise ValueError(f"hip precision: 'float32', got {precision!r}")
+        if precision not in ('float32', 'float16'):
+            raise ValueError(f"hip precision: 'float32' or 'float16', got {precision!r}")
         self.precision = precision
         self.predicted_error_db = None
         if T == 'auto':
@@ -247,6 +248,11 @@ class ImageFormer:
             self._form = make_ffbp_hip(plan, coll, wf=wf, window=None)
         else:
             self._form = make_ffbp_hip(plan, coll, wf=wf, window=win)
+        if precision == 'float16':
+            # the two-pass levels' range-filtered rows stored in float16 (scaled per call, no overflow): half their
+            # traffic for a rounding near -74 dB (the reference's cuda 'float16' keeps its level data in float16)
+            from . import _hip
+            _hip.check(_hip.lib().fsh_ffbp_set_precision(self._form.h, 1), 'ffbp_set_precision')
 
     def stage(self, S):
         """The hip former takes the history as it is: former(former.stage(S)) equals former(S). (fastsar/api.py:471-480)"""

@@ -23,10 +23,21 @@ namespace fsh {
 // ------------------------------------------------------------------------------------------------ prepare
 
 // x[p, k] = S[p, k] * wp[p] * wk[k] (wp == nullptr: copy); bad: the first pulse holding a NaN or inf
+// with xmax: the largest component magnitude of x as well (float bits compared as unsigned, one atomic per wave;
+// a NaN or inf, which the scan reports, is left out)
+__device__ __forceinline__ void note_max(float m, unsigned* xmax)
+{
+    if (!xmax) return;
+    for (int k = 32; k >= 1; k >>= 1) m = fmaxf(m, __shfl_xor(m, k, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(xmax, __float_as_uint(m));
+}
+
 __global__ __launch_bounds__(256) void k_prepare(const float2* __restrict__ S, int P, int K, const float* __restrict__ wp,
-                                                  const float* __restrict__ wk, float2* __restrict__ x, int* __restrict__ bad)
+                                                  const float* __restrict__ wk, float2* __restrict__ x, int* __restrict__ bad,
+                                                  unsigned* __restrict__ xmax)
 {
     const size_t n = (size_t)P * K;
+    float m = 0.f;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         float2 v = S[i];
         const int p = (int)(i / K);
@@ -36,7 +47,10 @@ __global__ __launch_bounds__(256) void k_prepare(const float2* __restrict__ S, i
             v.x *= g; v.y *= g;
         }
         x[i] = v;
+        const float a = fmaxf(fabsf(v.x), fabsf(v.y));
+        if (a < INFINITY) m = fmaxf(m, a);
     }
+    note_max(m, xmax);
 }
 
 // ------------------------------------------------------------------------------------------------ rot_fir_k
@@ -426,8 +440,11 @@ __global__ __launch_bounds__(256) void k_fir_p(const float2* __restrict__ y, int
 template <int NW, bool BATCH>
 __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__ y, int B, int P, int Ko,
                                                        const float* __restrict__ taps, int L, int pl, int D, int Po, int IC,
-                                                       float2* __restrict__ z)
+                                                       float2* __restrict__ z, const float* __restrict__ hs)
 {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const float hinv = hs ? hs[1] : 1.f;             // float16 input: its scale undone by hs[1]
+    const h2* yh = reinterpret_cast<const h2*>(y);
     extern __shared__ float tpz[];                   // [D * NW], zero beyond L (LDS form)
     if (!BATCH) {
         for (int r = threadIdx.x; r < D * NW; r += 256) tpz[r] = r < L ? taps[r] : 0.f;
@@ -438,6 +455,7 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
     const int b = (int)(col / Ko), j = (int)(col - (long)b * Ko);
     const int i0 = blockIdx.y * IC, i1 = min(Po, i0 + IC);
     const float2* yb = y + (size_t)b * P * Ko + j;
+    const h2* ybh = yh + (size_t)b * P * Ko + j;
     float2* zb = z + (size_t)b * Po * Ko + j;
     float2 acc[NW];
     #pragma unroll
@@ -449,7 +467,15 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
         if (!BATCH) {
             for (int ph = 0; ph < D; ++ph) {
                 const int p = pbase + ph;
-                const float2 v = (p >= 0 && p < P) ? yb[(size_t)p * Ko] : make_float2(0.f, 0.f);
+                float2 v = make_float2(0.f, 0.f);
+                if (p >= 0 && p < P) {
+                    if (hs) {
+                        const h2 w = ybh[(size_t)p * Ko];
+                        v = make_float2((float)w.x * hinv, (float)w.y * hinv);
+                    } else {
+                        v = yb[(size_t)p * Ko];
+                    }
+                }
                 #pragma unroll
                 for (int s = 0; s < NW; ++s) {
                     const float h = tpz[D * (NW - 1 - s) + ph];
@@ -465,7 +491,12 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
                 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int p = pbase + ph0 + u;
-                    v[u] = yb[(size_t)min(max(p, 0), P - 1) * Ko];
+                    if (hs) {
+                        const h2 w = ybh[(size_t)min(max(p, 0), P - 1) * Ko];
+                        v[u] = make_float2((float)w.x * hinv, (float)w.y * hinv);
+                    } else {
+                        v[u] = yb[(size_t)min(max(p, 0), P - 1) * Ko];
+                    }
                 }
                 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -824,11 +855,12 @@ static int lds_limit()
     return lim;
 }
 
-int launch_prepare(const float2* S, int P, int K, const float* wp, const float* wk, float2* x, int* bad, hipStream_t s)
+int launch_prepare(const float2* S, int P, int K, const float* wp, const float* wk, float2* x, int* bad, hipStream_t s,
+                   unsigned* xmax)
 {
     const size_t n = (size_t)P * K;
     const int blocks = (int)std::min<size_t>((n + 255) / 256, 1 << 16);
-    hipLaunchKernelGGL(k_prepare, dim3(blocks), dim3(256), 0, s, S, P, K, wp, wk, x, bad);
+    hipLaunchKernelGGL(k_prepare, dim3(blocks), dim3(256), 0, s, S, P, K, wp, wk, x, bad, xmax);
     FSH_TRY(hipGetLastError());
     return OK;
 }
@@ -865,9 +897,10 @@ __device__ __forceinline__ float2 decode_sample(const void* S, size_t i, const f
 template <int DT>
 __global__ __launch_bounds__(256) void k_prepare16(const void* __restrict__ S, int P, int K, const float* __restrict__ inv,
                                                     const float* __restrict__ wp, const float* __restrict__ wk,
-                                                    float2* __restrict__ x, int* __restrict__ bad)
+                                                    float2* __restrict__ x, int* __restrict__ bad, unsigned* __restrict__ xmax)
 {
     const size_t n = (size_t)P * K;
+    float m = 0.f;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int p = (int)(i / K);
         float2 v = decode_sample<DT>(S, i, inv, p);
@@ -877,22 +910,48 @@ __global__ __launch_bounds__(256) void k_prepare16(const void* __restrict__ S, i
             v.x *= g; v.y *= g;
         }
         x[i] = v;
+        const float a = fmaxf(fabsf(v.x), fabsf(v.y));
+        if (a < INFINITY) m = fmaxf(m, a);
     }
+    note_max(m, xmax);
+}
+
+// per level l: hs[2 l] = 2^(14 - ceil(log2(xmax gain[l]))), hs[2 l + 1] its inverse (gain[l] <= 0: 1, 1): a float16
+// store of level l's range-filtered rows, whose magnitudes gain[l] times the history's largest bounds, stays below
+// 2^14
+__global__ void k_half_scales(const unsigned* __restrict__ xmax, const float* __restrict__ gain, int n, float* __restrict__ hs)
+{
+    const int l = threadIdx.x;
+    if (l >= n) return;
+    const float m = __uint_as_float(*xmax) * gain[l];
+    float g = 1.f;
+    if (gain[l] > 0.f && m > 0.f && m < INFINITY) g = exp2f(14.f - ceilf(log2f(m)));
+    g = fminf(fmaxf(g, 0x1p-100f), 0x1p100f);
+    hs[2 * l] = g;
+    hs[2 * l + 1] = 1.f / g;
+}
+
+int launch_half_scales(const unsigned* xmax, const float* gain, int n, float* hs, hipStream_t s)
+{
+    if (n < 1 || n > 64) { set_error("half_scales: 1 to 64 levels"); return E_INVALID; }
+    hipLaunchKernelGGL(k_half_scales, dim3(1), dim3(64), 0, s, xmax, gain, n, hs);
+    FSH_TRY(hipGetLastError());
+    return OK;
 }
 
 int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, const float* wp, const float* wk, float2* x,
-                     int* bad, hipStream_t s)
+                     int* bad, hipStream_t s, unsigned* xmax)
 {
     const size_t n = (size_t)P * K;
     const int blocks = (int)std::min<size_t>((n + 255) / 256, 1 << 16);
     if (dtype == 1)
-        hipLaunchKernelGGL(k_prepare16<1>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad);
+        hipLaunchKernelGGL(k_prepare16<1>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax);
     else if (dtype == 2)
-        hipLaunchKernelGGL(k_prepare16<2>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad);
+        hipLaunchKernelGGL(k_prepare16<2>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax);
     else if (dtype == 3)
-        hipLaunchKernelGGL(k_prepare16<3>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad);
+        hipLaunchKernelGGL(k_prepare16<3>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax);
     else if (dtype == 4)
-        hipLaunchKernelGGL(k_prepare16<4>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad);
+        hipLaunchKernelGGL(k_prepare16<4>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax);
     else {
         set_error("prepare: unknown history type");
         return E_INVALID;
@@ -953,7 +1012,8 @@ int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, con
     return OK;
 }
 
-int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float2* z, hipStream_t s)
+int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float2* z, hipStream_t s,
+                 const float* hs)
 {
     if (B < 1 || P < 1 || Ko < 1 || L < 1 || D < 1 || Po < 1 || pl < 0) {
         set_error("fir_p: invalid shape");
@@ -961,6 +1021,10 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
     }
     const int NW = (L + D - 1) / D;
     const char* force = getenv("FSH_FIRP_KERNEL");
+    if (hs && (NW > 16 || (force && strcmp(force, "lds") == 0))) {
+        set_error("fir_p: a float16 input is read by the streaming kernel only (at most 16 taps per phase)");
+        return E_INVALID;
+    }
     if (NW <= 16 && !(force && strcmp(force, "lds") == 0)) {
         const int IC = std::min(Po, 128);
         dim3 grid((unsigned)(((long)B * Ko + 255) / 256), (Po + IC - 1) / IC);
@@ -968,8 +1032,8 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
         const size_t lds = batch ? 0 : (size_t)4 * D * NW;
 #define FSH_FP(NN)                                                                                                       \
     do {                                                                                                                 \
-        if (batch) hipLaunchKernelGGL((k_fir_p_stream<NN, true>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z); \
-        else hipLaunchKernelGGL((k_fir_p_stream<NN, false>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z);     \
+        if (batch) hipLaunchKernelGGL((k_fir_p_stream<NN, true>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs); \
+        else hipLaunchKernelGGL((k_fir_p_stream<NN, false>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs);     \
     } while (0)
         switch (NW) {
             case 1: FSH_FP(1); break;

@@ -9,6 +9,7 @@
 #include "../../include/fsh.h"
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <climits>
 #include <cstdio>
@@ -105,6 +106,7 @@ struct Level {
     float* hk_t = nullptr;         // polyphase tables of the fused form (fused levels only)
     float* hp_t = nullptr;
     bool fused = false;            // rotation and both filters in one pass (launch_level_fused)
+    std::vector<float> taps_k_host, taps_p_host;     // the taps, kept for the float16 scale bounds
     bool mfma = false;             // the same on the matrix cores (launch_level_mfma)
     long long below = 1;           // tiles per parent... children per first-level child at this level's input
 };
@@ -130,6 +132,11 @@ struct Former {
     float2* hist = nullptr;        // the history as uploaded (host input)
     float2* x = nullptr;           // windowed history
     float* pinv = nullptr;         // a float16 history's per-pulse inverse scales, uploaded (host input)
+    // float16 storage of the two-pass levels' range-filtered rows (fsh_ffbp_set_precision)
+    bool half_mid = false;
+    unsigned* xmax = nullptr;      // the windowed history's largest component magnitude (float bits)
+    float* hgain = nullptr;        // per level: the bound of its range-filtered rows over that magnitude
+    float* hsc = nullptr;          // per level: the store scale and its inverse
     int* bad = nullptr;
     float2* image = nullptr;
     // per group
@@ -169,7 +176,7 @@ struct Former {
         }
         lv.clear();
         dev_free(one); dev_free(wp); dev_free(wk); dev_free(fu); dev_free(fr0); dev_free(fcen); dev_free(tile_xy);
-        dev_free(w0); dev_free(gx); dev_free(gyr); dev_free(hist); dev_free(x); dev_free(pinv); dev_free(bad); dev_free(image);
+        dev_free(w0); dev_free(gx); dev_free(gyr); dev_free(hist); dev_free(x); dev_free(pinv); dev_free(xmax); dev_free(hgain); dev_free(hsc); dev_free(bad); dev_free(image);
         if (ev[0]) (void)hipEventDestroy(ev[0]);
         if (ev[1]) (void)hipEventDestroy(ev[1]);
         if (stream) (void)hipStreamDestroy(stream);
@@ -305,8 +312,17 @@ struct Former {
                 continue;
             }
             mark_begin();
-            if ((st = launch_rot_fir_k(cur, (int)parents, h.P, h.K, c0, sl, ch, h.Dk > 1 ? L.taps_k : one, h.Dk > 1 ? h.Lk : 1,
-                                       h.Dk > 1 ? h.plk : 0, h.Dk, h.Ko, h.kc, buf[yb], stream)))
+            // the low-precision form: the range-filtered rows of a decimated-in-pulse level in float16 where the row
+            // form of the matrix-core filter takes the level and the streaming pulse filter reads it back
+            const float* hs = nullptr;
+            if (half_mid && h.Dk > 1 && h.Dp > 1 && (h.Lp + h.Dp - 1) / h.Dp <= 16 && !getenv("FSH_FIRP_KERNEL")) {
+                st = launch_rot_fir_k_mf2(cur, (int)parents, h.P, h.K, c0, sl, ch, L.taps_k, h.Lk, h.plk, h.Dk, h.Ko, h.kc,
+                                          buf[yb], stream, hsc + 2 * l);
+                if (st == OK) hs = hsc + 2 * l;
+                else if (st != E_INVALID) return st;
+            }
+            if (!hs && (st = launch_rot_fir_k(cur, (int)parents, h.P, h.K, c0, sl, ch, h.Dk > 1 ? L.taps_k : one,
+                                              h.Dk > 1 ? h.Lk : 1, h.Dk > 1 ? h.plk : 0, h.Dk, h.Ko, h.kc, buf[yb], stream)))
                 return st;
             mark_end(ST_ROT);
             cur = buf[yb];
@@ -314,7 +330,7 @@ struct Former {
             if (h.Dp > 1) {
                 mark_begin();
                 if ((st = launch_fir_p(buf[yb], (int)(parents * ch), h.P, h.Ko, L.taps_p, h.Lp, h.plp, h.Dp, h.Po, buf[1 - yb],
-                                       stream)))
+                                       stream, hs)))
                     return st;
                 mark_end(ST_FIRP);
                 cur = buf[1 - yb];
@@ -533,6 +549,8 @@ int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, voi
         L.h.taps_k = L.h.taps_p = L.h.c0 = L.h.sl = nullptr;
         L.h.u = L.h.r0 = L.h.d = L.h.ref = nullptr;
         if ((st = upload(&L.taps_k, h.taps_k, h.Dk > 1 ? (size_t)h.Lk : 0))) return fail(st);
+        if (h.Dk > 1) L.taps_k_host.assign(h.taps_k, h.taps_k + h.Lk);
+        if (h.Dp > 1) L.taps_p_host.assign(h.taps_p, h.taps_p + h.Lp);
         if ((st = upload(&L.taps_p, h.taps_p, h.Dp > 1 ? (size_t)h.Lp : 0))) return fail(st);
         if (h.c0) {
             const size_t n = (size_t)h.nparents * h.C * h.P;
@@ -659,11 +677,13 @@ static int ffbp_form_impl(Former* f, const void* S, int dtype, const float* puls
     }
     f->mark_begin();
     if ((st = launch_fill_int(f->bad, 1, INT_MAX, f->stream))) return st;
+    if (f->half_mid && (st = launch_fill_int((int*)f->xmax, 1, 0, f->stream))) return st;
     if (dtype == FSH_C64) {
-        if ((st = launch_prepare((const float2*)src, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream))) return st;
-    } else if ((st = launch_prepare16(src, dtype, inv, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream))) {
+        if ((st = launch_prepare((const float2*)src, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream, f->xmax))) return st;
+    } else if ((st = launch_prepare16(src, dtype, inv, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream, f->xmax))) {
         return st;
     }
+    if (f->half_mid && (st = launch_half_scales(f->xmax, f->hgain, (int)f->lv.size(), f->hsc, f->stream))) return st;
     f->mark_end(ST_PREP);
     int badh = INT_MAX;
     FSH_TRY(hipMemcpyAsync(&badh, f->bad, sizeof(int), hipMemcpyDeviceToHost, f->stream));
@@ -714,6 +734,38 @@ int fsh_ffbp_form_typed(void* handle, const void* S, int dtype, const float* pul
     return ffbp_form_impl(static_cast<Former*>(handle), S, dtype, pulse_scale, S_on_device, image, image_on_device, bad_pulse);
 }
 
+int fsh_ffbp_set_precision(void* handle, int half_intermediates)
+{
+    if (!handle) { set_error("ffbp_set_precision: null handle"); return E_INVALID; }
+    Former* f = static_cast<Former*>(handle);
+    if (!half_intermediates) {
+        f->half_mid = false;
+        return OK;
+    }
+    const int n = (int)f->lv.size();
+    if (!f->xmax) {
+        int st;
+        if ((st = dev_alloc((void**)&f->xmax, sizeof(unsigned))) || (st = dev_alloc((void**)&f->hsc, 2 * n * sizeof(float))) ||
+            (st = dev_alloc((void**)&f->hgain, n * sizeof(float))))
+            return st;
+        // the bound of level l's range-filtered rows over the windowed history's largest component: the L1 norms of
+        // every filter before them (the rotations keep magnitudes); two components, so times sqrt 2 for the modulus
+        std::vector<float> g(n, 0.f);
+        double acc = std::sqrt(2.0);
+        for (int l = 0; l < n; ++l) {
+            const fsh_level_desc& h = f->lv[l].h;
+            double nk = 1.0, np = 1.0;
+            if (h.Dk > 1) { nk = 0; for (int r = 0; r < h.Lk; ++r) nk += std::fabs(f->lv[l].taps_k_host[r]); }
+            if (h.Dp > 1) { np = 0; for (int r = 0; r < h.Lp; ++r) np += std::fabs(f->lv[l].taps_p_host[r]); }
+            g[l] = (float)(acc * nk);
+            acc *= nk * np;
+        }
+        FSH_TRY(hipMemcpy(f->hgain, g.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    f->half_mid = true;
+    return OK;
+}
+
 int fsh_ffbp_enqueue(void* handle, const void* S, void* image, void* stream)
 {
     if (!handle || !S || !image) { set_error("ffbp_enqueue: null argument"); return E_INVALID; }
@@ -730,7 +782,9 @@ int fsh_ffbp_enqueue(void* handle, const void* S, void* image, void* stream)
     f->stream = static_cast<hipStream_t>(stream);
     f->image = static_cast<float2*>(image);
     int st = launch_fill_int(f->bad, 1, INT_MAX, f->stream);
-    if (!st) st = launch_prepare(static_cast<const float2*>(S), p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream);
+    if (!st && f->half_mid) st = launch_fill_int((int*)f->xmax, 1, 0, f->stream);
+    if (!st) st = launch_prepare(static_cast<const float2*>(S), p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream, f->xmax);
+    if (!st && f->half_mid) st = launch_half_scales(f->xmax, f->hgain, (int)f->lv.size(), f->hsc, f->stream);
     const int G = f->lv[0].h.C;
     for (int g0 = 0; !st && g0 < G; g0 += f->ng) st = f->run_group(g0, std::min(f->ng, G - g0));
     f->stream = own;

@@ -5,19 +5,23 @@
 
 namespace fsh {
 
-int launch_prepare(const float2* S, int P, int K, const float* wp, const float* wk, float2* x, int* bad, hipStream_t s);
+int launch_prepare(const float2* S, int P, int K, const float* wp, const float* wk, float2* x, int* bad, hipStream_t s,
+                   unsigned* xmax = nullptr);
+int launch_half_scales(const unsigned* xmax, const float* gain, int n, float* hs, hipStream_t s);
 // the history types of fsh_ffbp_form_typed and fsh_exact_form_typed with their scale rule: OK or E_INVALID (message set)
 int check_history_type(int dtype, const void* pulse_scale);
 int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, const float* wp, const float* wk, float2* x,
-                     int* bad, hipStream_t s);
+                     int* bad, hipStream_t s, unsigned* xmax = nullptr);
 int launch_fill_int(int* p, int n, int value, hipStream_t s);
 int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
                      int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s);
-// the row form of the matrix-core range filter (range_mfma.hip); E_INVALID when the taps do not fit it
+// the row form of the matrix-core range filter (range_mfma.hip); E_INVALID when the taps do not fit it. hs: store
+// the output as float16 pairs times hs[0] (4 bytes a sample) instead of float2
 int launch_rot_fir_k_mf2(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
-                         int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s);
+                         int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s, const float* hs = nullptr);
+// hs: the input is float16 pairs, times hs[1] when read (the streaming kernel only)
 int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float2* z,
-                 hipStream_t s);
+                 hipStream_t s, const float* hs = nullptr);
 // the rotation and both filters of a level in one pass, for short children (fused_kernels.hip); hk_t and hp_t are the
 // polyphase tables that launch_polyphase builds from the taps (h == nullptr: the unit tap of an undecimated axis)
 int fused_nw(int L, int D);

@@ -57,7 +57,7 @@ template <int KS>
 __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restrict__ x, int P, int K, const float* __restrict__ c0,
                                                         const float* __restrict__ sl, int Cn, const float* __restrict__ taps,
                                                         int L, int pl, int D, int Ko, double kc, int NWp, int NPH, int PS,
-                                                        int MC, float2* __restrict__ y)
+                                                        int MC, float2* __restrict__ y, const float* __restrict__ hs)
 {
     extern __shared__ unsigned int lds_u[];
     unsigned int* XH = lds_u;                                  // [NPH][PS]
@@ -194,7 +194,14 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
                 const int cl = q + 2 * it, cc = nt * 16 + cl;
                 if (cc < Cn && j < Ko) {
                     const float2 v = *reinterpret_cast<const float2*>(stg + cl * 66 + 2 * col);
-                    y[(((size_t)n * Cn + cc) * P + p) * Ko + j] = v;
+                    const size_t o = (((size_t)n * Cn + cc) * P + p) * Ko + j;
+                    if (hs) {                          // float16 storage, scaled by hs[0] (a power of two)
+                        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+                        const float g = hs[0];
+                        reinterpret_cast<h2*>(y)[o] = h2{(_Float16)(v.x * g), (_Float16)(v.y * g)};
+                    } else {
+                        y[o] = v;
+                    }
                 }
             }
             wave_lds_sync();                                   // the next tile's writes to stg come after these reads
@@ -225,7 +232,7 @@ static bool mf2_shape(int L, int D, int Ko, int& NWp, int& KS, int& NPH, int& PS
 }
 
 int launch_rot_fir_k_mf2(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
-                         int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s)
+                         int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s, const float* hs)
 {
     int NWp, KS, NPH, PS, MC, chunks;
     size_t lds;
@@ -236,9 +243,11 @@ int launch_rot_fir_k_mf2(const float2* x, int Np, int P, int K, const float* c0,
         const float2* xs = x + (size_t)n0 * P * K;
         const float* c0s = c0 + (size_t)n0 * Cn * P;
         const float* sls = sl + (size_t)n0 * Cn * P;
-        float2* ys = y + (size_t)n0 * Cn * P * Ko;
+        // a float16 output counts 4 bytes a sample: the chunk's offset in float2 units is halved
+        float2* ys = hs ? reinterpret_cast<float2*>(reinterpret_cast<char*>(y) + (size_t)n0 * Cn * P * Ko * 4)
+                        : y + (size_t)n0 * Cn * P * Ko;
 #define FSH_M2(KK) hipLaunchKernelGGL(k_rot_fir_k_mf2<KK>, grid, dim3(256), lds, s, xs, P, K, c0s, sls, Cn, taps, L, pl, D, Ko, \
-                                      kc, NWp, NPH, PS, MC, ys)
+                                      kc, NWp, NPH, PS, MC, ys, hs)
         switch (KS) {
             case 1: FSH_M2(1); break;   case 2: FSH_M2(2); break;   case 3: FSH_M2(3); break;   case 4: FSH_M2(4); break;
             case 5: FSH_M2(5); break;   case 6: FSH_M2(6); break;   case 7: FSH_M2(7); break;   case 8: FSH_M2(8); break;

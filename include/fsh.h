@@ -157,6 +157,11 @@ int fsh_ffbp_form(void* handle, const void* S, int S_on_device, void* image, int
 #define FSH_CI8 4
 int fsh_ffbp_form_typed(void* handle, const void* S, int dtype, const float* pulse_scale, int S_on_device, void* image,
                         int image_on_device, int* bad_pulse);
+/* The low-precision form (the reference's cuda precision 'float16', fastsar/api.py:424-431, keeps its level data in
+ * float16): half_intermediates = 1 stores the range-filtered rows of the two-pass levels as float16 pairs, scaled
+ * per call by a power of two from the history's largest component and the filters' L1 norms (no overflow), which
+ * halves their traffic; 0 (default) keeps float32. */
+int fsh_ffbp_set_precision(void* handle, int half_intermediates);
 int fsh_ffbp_set_group(void* handle, int ng);
 /* The group used by the last call and the device bytes the former holds. */
 int fsh_ffbp_info(void* handle, int* ng_used, size_t* device_bytes);

@@ -91,3 +91,23 @@ def test_cphd_integer_samples(c1, kind, amp, algo):
     sig = pr.CphdSignal(raw, a)
     assert np.array_equal(f(sig), want)
     assert np.array_equal(f(sig.to_device()), want)
+
+
+def test_float16_intermediates(c1):
+    """precision='float16': the two-pass levels' range-filtered rows stored as scaled float16 pairs. The image moves
+    by the storage's rounding only (well under the float32 image's own error) and its error against the FP64 oracle
+    stays within 0.5 dB"""
+    col, S, grid = c1
+    wp, wk = oracle.window(*S.shape)
+    ref = oracle.bp_points(S * wp[:, None] * wk[None, :], col.ant, col.fmin, col.df, oracle.plane_points(**grid),
+                           upsample=64, interp='cubic')
+    kw = dict(T=8, levels=4, atten=80.0)
+    a = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, **kw)(S)
+    f = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, **kw, precision='float16')
+    b = f(S)
+    d, ea, eb = oracle.plain_db(b, a), oracle.rel_db(a, ref), oracle.rel_db(b, ref)
+    print(f'float16 intermediates: {d:.1f} dB from float32; oracle {eb:.1f} (float32 {ea:.1f}) dB')
+    assert -100 < d < -65 and eb < ea + 0.5
+    # a history 2^40 times larger: the per-call scale keeps the float16 rows in range
+    big = f((S * np.float32(2.0 ** 40)).astype(np.complex64))
+    assert oracle.plain_db(big / np.float32(2.0 ** 40), b) < -100

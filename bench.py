@@ -283,6 +283,7 @@ def main():
     ap.add_argument('--batch-target', type=float, default=-60.0, help='predicted final-stage error the autotuner meets (dB)')
     ap.add_argument('--batch-atten', type=float, default=80.0, help='stopband attenuation of the batch formers\' filters (dB)')
     ap.add_argument('--taps', default=None, help='learned decimation filters (an .npz of fastsar_amd.learned) for the image')
+    ap.add_argument('--no-learned', action='store_true', help='skip the learned form beside the plain one')
     ap.add_argument('--train-taps', type=int, default=0, help='train filters of this many taps per phase on 2 scenes of '
                     'the workload first (seeds 11, 12), saved to gpurun_out/learned_T<n>.npz, and form with them')
     ap.add_argument('--precision', default='float32', choices=['float32', 'float16'],
@@ -442,7 +443,7 @@ def main():
     # of this workload (fastsar_amd/data/learned_c2_T8.npz), same plan otherwise
     learned_form = None
     lpath = os.path.join(ROOT, 'fastsar_amd', 'data', f'learned_c2_T8.npz')
-    if (not args.small and taps is None and args.T == 8 and args.levels == 4 and args.atten == 80.0
+    if (not args.small and not args.no_learned and taps is None and args.T == 8 and args.levels == 4 and args.atten == 80.0
             and args.history == 'complex64' and args.precision == 'float32' and os.path.exists(lpath)):
         from fastsar_amd import learned
         lt = learned.LearnedTaps.load(lpath)

@@ -241,9 +241,13 @@ bool level_fused_shape(int C, int P, int K, int Lk, int Dk, int Ko, int Lp, int 
         return false;
     CB = std::min(C, 256 / Ko);
     const int KS = K | 1, KoS = Ko | 1;
-    const size_t budget = 78 * 1024;
+    // 39 KB of rows a block: four blocks a CU (16 waves) hide the row loads' latency (the 12288 x 8192 image's level 3:
+    // 53 ms against 69 ms at 78 KB, two blocks a CU); up to 78 KB where 39 KB would hold fewer than 16 rows
+    size_t budget = 39 * 1024;
+    if (const char* e = getenv("FSH_FUSED_LDS_KB")) budget = (size_t)std::max(8, std::min(78, atoi(e))) * 1024;   // sweeps
     const size_t per_row = 8 * ((size_t)KS + (size_t)CB * KoS);
     R = (int)std::min<size_t>(budget / per_row, (size_t)(256 / CB));
+    if (R < 16 && R < P && !getenv("FSH_FUSED_LDS_KB")) R = (int)std::min<size_t>(78 * 1024 / per_row, (size_t)(256 / CB));
     R = std::min(R, P);
     if (R < 8 && R < P) return false;
     lds = per_row * R;

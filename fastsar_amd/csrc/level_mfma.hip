@@ -375,7 +375,9 @@ bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int
     // segments of output pulses: enough blocks to fill the device, a halo below a fifth of the rows
     const long blocks0 = (long)Np * ((C + CB - 1) / CB) * njr;
     nseg = 1;
-    while (nseg < Po && blocks0 * nseg < 2048 && (long)Dp * ((Po + 2 * nseg - 1) / (2 * nseg)) >= 5L * Lp) nseg *= 2;
+    long want = 2048;
+    if (const char* e = getenv("FSH_LMFMA_BLOCKS")) want = std::max(1, atoi(e));     // sweeps
+    while (nseg < Po && blocks0 * nseg < want && (long)Dp * ((Po + 2 * nseg - 1) / (2 * nseg)) >= 5L * Lp) nseg *= 2;
     IS = (Po + nseg - 1) / nseg;
     nseg = (Po + IS - 1) / IS;
     return true;

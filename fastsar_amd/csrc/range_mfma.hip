@@ -221,7 +221,9 @@ static bool mf2_shape(int L, int D, int Ko, int& NWp, int& KS, int& NPH, int& PS
     const int MT = (Ko + 31) / 32;
     // window planes within ~36 KB so that three blocks share a CU (with the transpose buffers and the tap table)
     const size_t fixed = (size_t)4 * 4 * STG + sizeof(PairTab) * 4 * KS;
-    const size_t budget = 52 * 1024 - fixed;
+    size_t total = 52 * 1024;
+    if (const char* e = getenv("FSH_MF2_LDS_KB")) total = (size_t)std::max(24, std::min(156, atoi(e))) * 1024;   // sweeps
+    const size_t budget = total - fixed;
     MC = std::max(1, (int)((budget / (8 * (size_t)NPH) - NWp - 1) / 32));
     MC = std::min(MC, MT);
     chunks = (MT + MC - 1) / MC;

@@ -71,6 +71,7 @@ def lib():
         'fsh_ffbp_form': ([vp, vp, c_int, vp, c_int, P_i32], c_int),
         'fsh_ffbp_form_typed': ([vp, vp, c_int, vp, c_int, vp, c_int, P_i32], c_int),
         'fsh_ffbp_set_precision': ([vp, c_int], c_int),
+        'fsh_ffbp_set_ramp': ([vp, vp], c_int),
         'fsh_ffbp_set_group': ([vp, c_int], c_int),
         'fsh_ffbp_info': ([vp, P_i32, ctypes.POINTER(c_size_t)], c_int),
         'fsh_ffbp_stage_ms': ([vp, P_f32, c_int], c_int),

@@ -14,6 +14,7 @@ samples of the windowed history times exp(+j 4 pi f dR / c).
 Backends: 'hip' (hand-written HIP kernels on an AMD Instinct GPU, gfx950) and 'auto' (the same). The reference's
 other backend names are recognised and refused with the reason.
 """
+import ctypes
 import math
 import warnings
 
@@ -175,6 +176,7 @@ class ImageFormer:
         if not (np.isfinite(fmin) and np.isfinite(df) and fmin > 0 and df > 0):
             raise ValueError(f'fmin and df must be positive frequencies in Hz, got {fmin!r} and {df!r}')
         self.window = window
+        self._fmin, self._df = float(fmin), float(df)
         col = Collect(fmin=float(fmin), df=float(df), K=self.K, ant=self.ant, res=0.5)
         nx, ny, e1, e2 = _check_grid(nx, ny, spx, spy, e1, e2)
         self.backend = _backend(backend)
@@ -253,6 +255,24 @@ class ImageFormer:
             # traffic for a rounding near -74 dB (the reference's cuda 'float16' keeps its level data in float16)
             from . import _hip
             _hip.check(_hip.lib().fsh_ffbp_set_precision(self._form.h, 1), 'ffbp_set_precision')
+
+    def rereference(self, coef, sign=-1):
+        """Apply the phase re-reference of a CPHD read on the device as each call takes its history: sample k of
+        pulse p times exp(sign 2 pi i f_k coef[p]), f_k = fmin + k df (fastsar/io.py:384-420; fastsar/cphd.py:401 moves
+        the reference to the grid centre with coef = 2 dref / c, sign -1). coef None removes it."""
+        from . import _hip
+        if self._gate is not None:
+            raise ValueError('a gated former takes its history re-referenced on the host')
+        if coef is None:
+            _hip.check(_hip.lib().fsh_ffbp_set_ramp(self._form.h, None), 'ffbp_set_ramp')
+            return
+        coef = np.asarray(coef, np.float64)
+        if coef.shape != (self.P,) or not np.all(np.isfinite(coef)):
+            raise ValueError(f'coef must hold one finite value per pulse ({self.P}), got shape {coef.shape}')
+        if sign not in (-1, 1):
+            raise ValueError('sign must be +1 or -1')
+        ab = np.ascontiguousarray(np.stack([sign * self._fmin * coef, sign * self._df * coef], 1))
+        _hip.check(_hip.lib().fsh_ffbp_set_ramp(self._form.h, ab.ctypes.data_as(ctypes.c_void_p)), 'ffbp_set_ramp')
 
     def stage(self, S):
         """The hip former takes the history as it is: former(former.stage(S)) equals former(S). (fastsar/api.py:471-480)"""

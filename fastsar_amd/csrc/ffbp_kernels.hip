@@ -32,9 +32,20 @@ __device__ __forceinline__ void note_max(float m, unsigned* xmax)
     if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(xmax, __float_as_uint(m));
 }
 
+// the phase re-reference of a CPHD read (fastsar/io.py:384-420, fastsar/cphd.py:391-401) applied on the way in:
+// sample k of pulse p times exp(2 pi i (a_p + b_p k)), the phase in cycles formed in float64 and wrapped
+__device__ __forceinline__ float2 apply_ramp(float2 v, const double2* ramp, int p, int k)
+{
+    if (!ramp) return v;
+    const double2 ab = ramp[p];
+    float cs, sn;
+    cis_cyc(fma((double)k, ab.y, ab.x), cs, sn);
+    return make_float2(fmaf(v.x, cs, -(v.y * sn)), fmaf(v.x, sn, v.y * cs));
+}
+
 __global__ __launch_bounds__(256) void k_prepare(const float2* __restrict__ S, int P, int K, const float* __restrict__ wp,
                                                   const float* __restrict__ wk, float2* __restrict__ x, int* __restrict__ bad,
-                                                  unsigned* __restrict__ xmax)
+                                                  unsigned* __restrict__ xmax, const double2* __restrict__ ramp)
 {
     const size_t n = (size_t)P * K;
     float m = 0.f;
@@ -42,6 +53,7 @@ __global__ __launch_bounds__(256) void k_prepare(const float2* __restrict__ S, i
         float2 v = S[i];
         const int p = (int)(i / K);
         if (!(isfinite(v.x) && isfinite(v.y))) atomicMin(bad, p);
+        v = apply_ramp(v, ramp, p, (int)(i - (size_t)p * K));
         if (wp) {
             const float g = wp[p] * wk[i - (size_t)p * K];
             v.x *= g; v.y *= g;
@@ -856,11 +868,11 @@ static int lds_limit()
 }
 
 int launch_prepare(const float2* S, int P, int K, const float* wp, const float* wk, float2* x, int* bad, hipStream_t s,
-                   unsigned* xmax)
+                   unsigned* xmax, const double* ramp)
 {
     const size_t n = (size_t)P * K;
     const int blocks = (int)std::min<size_t>((n + 255) / 256, 1 << 16);
-    hipLaunchKernelGGL(k_prepare, dim3(blocks), dim3(256), 0, s, S, P, K, wp, wk, x, bad, xmax);
+    hipLaunchKernelGGL(k_prepare, dim3(blocks), dim3(256), 0, s, S, P, K, wp, wk, x, bad, xmax, (const double2*)ramp);
     FSH_TRY(hipGetLastError());
     return OK;
 }
@@ -897,7 +909,8 @@ __device__ __forceinline__ float2 decode_sample(const void* S, size_t i, const f
 template <int DT>
 __global__ __launch_bounds__(256) void k_prepare16(const void* __restrict__ S, int P, int K, const float* __restrict__ inv,
                                                     const float* __restrict__ wp, const float* __restrict__ wk,
-                                                    float2* __restrict__ x, int* __restrict__ bad, unsigned* __restrict__ xmax)
+                                                    float2* __restrict__ x, int* __restrict__ bad, unsigned* __restrict__ xmax,
+                                                    const double2* __restrict__ ramp)
 {
     const size_t n = (size_t)P * K;
     float m = 0.f;
@@ -905,6 +918,7 @@ __global__ __launch_bounds__(256) void k_prepare16(const void* __restrict__ S, i
         const int p = (int)(i / K);
         float2 v = decode_sample<DT>(S, i, inv, p);
         if (!(isfinite(v.x) && isfinite(v.y))) atomicMin(bad, p);
+        v = apply_ramp(v, ramp, p, (int)(i - (size_t)p * K));
         if (wp) {
             const float g = wp[p] * wk[i - (size_t)p * K];
             v.x *= g; v.y *= g;
@@ -940,18 +954,22 @@ int launch_half_scales(const unsigned* xmax, const float* gain, int n, float* hs
 }
 
 int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, const float* wp, const float* wk, float2* x,
-                     int* bad, hipStream_t s, unsigned* xmax)
+                     int* bad, hipStream_t s, unsigned* xmax, const double* ramp)
 {
     const size_t n = (size_t)P * K;
     const int blocks = (int)std::min<size_t>((n + 255) / 256, 1 << 16);
     if (dtype == 1)
-        hipLaunchKernelGGL(k_prepare16<1>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax);
+        hipLaunchKernelGGL(k_prepare16<1>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax,
+                           (const double2*)ramp);
     else if (dtype == 2)
-        hipLaunchKernelGGL(k_prepare16<2>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax);
+        hipLaunchKernelGGL(k_prepare16<2>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax,
+                           (const double2*)ramp);
     else if (dtype == 3)
-        hipLaunchKernelGGL(k_prepare16<3>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax);
+        hipLaunchKernelGGL(k_prepare16<3>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax,
+                           (const double2*)ramp);
     else if (dtype == 4)
-        hipLaunchKernelGGL(k_prepare16<4>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax);
+        hipLaunchKernelGGL(k_prepare16<4>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax,
+                           (const double2*)ramp);
     else {
         set_error("prepare: unknown history type");
         return E_INVALID;

@@ -137,6 +137,8 @@ struct Former {
     unsigned* xmax = nullptr;      // the windowed history's largest component magnitude (float bits)
     float* hgain = nullptr;        // per level: the bound of its range-filtered rows over that magnitude
     float* hsc = nullptr;          // per level: the store scale and its inverse
+    double* ramp = nullptr;        // [P] (a, b) cycles: the phase re-reference applied on the way in (fsh_ffbp_set_ramp)
+    bool use_ramp = false;
     int* bad = nullptr;
     float2* image = nullptr;
     // per group
@@ -176,7 +178,7 @@ struct Former {
         }
         lv.clear();
         dev_free(one); dev_free(wp); dev_free(wk); dev_free(fu); dev_free(fr0); dev_free(fcen); dev_free(tile_xy);
-        dev_free(w0); dev_free(gx); dev_free(gyr); dev_free(hist); dev_free(x); dev_free(pinv); dev_free(xmax); dev_free(hgain); dev_free(hsc); dev_free(bad); dev_free(image);
+        dev_free(w0); dev_free(gx); dev_free(gyr); dev_free(hist); dev_free(x); dev_free(pinv); dev_free(xmax); dev_free(hgain); dev_free(hsc); dev_free(ramp); dev_free(bad); dev_free(image);
         if (ev[0]) (void)hipEventDestroy(ev[0]);
         if (ev[1]) (void)hipEventDestroy(ev[1]);
         if (stream) (void)hipStreamDestroy(stream);
@@ -678,9 +680,10 @@ static int ffbp_form_impl(Former* f, const void* S, int dtype, const float* puls
     f->mark_begin();
     if ((st = launch_fill_int(f->bad, 1, INT_MAX, f->stream))) return st;
     if (f->half_mid && (st = launch_fill_int((int*)f->xmax, 1, 0, f->stream))) return st;
+    const double* rp = f->use_ramp ? f->ramp : nullptr;
     if (dtype == FSH_C64) {
-        if ((st = launch_prepare((const float2*)src, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream, f->xmax))) return st;
-    } else if ((st = launch_prepare16(src, dtype, inv, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream, f->xmax))) {
+        if ((st = launch_prepare((const float2*)src, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream, f->xmax, rp))) return st;
+    } else if ((st = launch_prepare16(src, dtype, inv, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream, f->xmax, rp))) {
         return st;
     }
     if (f->half_mid && (st = launch_half_scales(f->xmax, f->hgain, (int)f->lv.size(), f->hsc, f->stream))) return st;
@@ -734,6 +737,21 @@ int fsh_ffbp_form_typed(void* handle, const void* S, int dtype, const float* pul
     return ffbp_form_impl(static_cast<Former*>(handle), S, dtype, pulse_scale, S_on_device, image, image_on_device, bad_pulse);
 }
 
+int fsh_ffbp_set_ramp(void* handle, const double* ab)
+{
+    if (!handle) { set_error("ffbp_set_ramp: null handle"); return E_INVALID; }
+    Former* f = static_cast<Former*>(handle);
+    if (!ab) {
+        f->use_ramp = false;
+        return OK;
+    }
+    int st;
+    if (!f->ramp && (st = dev_alloc((void**)&f->ramp, (size_t)f->p.P * 2 * sizeof(double)))) return st;
+    FSH_TRY(hipMemcpy(f->ramp, ab, (size_t)f->p.P * 2 * sizeof(double), hipMemcpyHostToDevice));
+    f->use_ramp = true;
+    return OK;
+}
+
 int fsh_ffbp_set_precision(void* handle, int half_intermediates)
 {
     if (!handle) { set_error("ffbp_set_precision: null handle"); return E_INVALID; }
@@ -783,7 +801,8 @@ int fsh_ffbp_enqueue(void* handle, const void* S, void* image, void* stream)
     f->image = static_cast<float2*>(image);
     int st = launch_fill_int(f->bad, 1, INT_MAX, f->stream);
     if (!st && f->half_mid) st = launch_fill_int((int*)f->xmax, 1, 0, f->stream);
-    if (!st) st = launch_prepare(static_cast<const float2*>(S), p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream, f->xmax);
+    if (!st) st = launch_prepare(static_cast<const float2*>(S), p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream, f->xmax,
+                                 f->use_ramp ? f->ramp : nullptr);
     if (!st && f->half_mid) st = launch_half_scales(f->xmax, f->hgain, (int)f->lv.size(), f->hsc, f->stream);
     const int G = f->lv[0].h.C;
     for (int g0 = 0; !st && g0 < G; g0 += f->ng) st = f->run_group(g0, std::min(f->ng, G - g0));

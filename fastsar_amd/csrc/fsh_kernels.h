@@ -5,13 +5,14 @@
 
 namespace fsh {
 
+// ramp: [P] (a, b) in cycles, the phase re-reference exp(2 pi i (a + b k)) applied to every sample before the window
 int launch_prepare(const float2* S, int P, int K, const float* wp, const float* wk, float2* x, int* bad, hipStream_t s,
-                   unsigned* xmax = nullptr);
+                   unsigned* xmax = nullptr, const double* ramp = nullptr);
 int launch_half_scales(const unsigned* xmax, const float* gain, int n, float* hs, hipStream_t s);
 // the history types of fsh_ffbp_form_typed and fsh_exact_form_typed with their scale rule: OK or E_INVALID (message set)
 int check_history_type(int dtype, const void* pulse_scale);
 int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, const float* wp, const float* wk, float2* x,
-                     int* bad, hipStream_t s, unsigned* xmax = nullptr);
+                     int* bad, hipStream_t s, unsigned* xmax = nullptr, const double* ramp = nullptr);
 int launch_fill_int(int* p, int n, int value, hipStream_t s);
 int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
                      int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s);

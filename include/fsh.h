@@ -162,6 +162,10 @@ int fsh_ffbp_form_typed(void* handle, const void* S, int dtype, const float* pul
  * per call by a power of two from the history's largest component and the filters' L1 norms (no overflow), which
  * halves their traffic; 0 (default) keeps float32. */
 int fsh_ffbp_set_precision(void* handle, int half_intermediates);
+/* The phase re-reference of a CPHD read (fastsar/io.py:384-420, called at fastsar/cphd.py:401 with
+ * a = -f0 coef, b = -df coef, coef = 2 dref / c), applied on the device as each call takes its history: ab [P, 2]
+ * float64 (host) in cycles, sample k of pulse p times exp(2 pi i (a_p + b_p k)) before the window; NULL removes it. */
+int fsh_ffbp_set_ramp(void* handle, const double* ab);
 int fsh_ffbp_set_group(void* handle, int ng);
 /* The group used by the last call and the device bytes the former holds. */
 int fsh_ffbp_info(void* handle, int* ng_used, size_t* device_bytes);

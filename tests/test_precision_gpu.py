@@ -111,3 +111,24 @@ def test_float16_intermediates(c1):
     # a history 2^40 times larger: the per-call scale keeps the float16 rows in range
     big = f((S * np.float32(2.0 ** 40)).astype(np.complex64))
     assert oracle.plain_db(big / np.float32(2.0 ** 40), b) < -100
+
+
+def test_phase_rereference_on_device(c1):
+    """former.rereference(coef): the CPHD read's phase re-reference (fastsar/io.py:384-420) applied on the device
+    equals the history multiplied on the host in float64 (float32 rounding only); None removes it"""
+    col, S, grid = c1
+    rng = np.random.default_rng(9)
+    coef = 2 * rng.uniform(-3.0, 3.0, col.Np) / sim.C           # reference moves of up to 3 m
+    f = col.fmin + col.df * np.arange(col.K)
+    Sr = (S * np.exp(-2j * np.pi * f[None, :] * coef[:, None])).astype(np.complex64)
+    former = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid)
+    want = former(Sr)
+    former.rereference(coef)
+    got = former(S)
+    e = oracle.plain_db(got, want)
+    print(f're-reference on the device: {e:.1f} dB from the host-side re-reference')
+    assert e < -110
+    former.rereference(None)
+    assert np.array_equal(former(Sr), want)
+    with pytest.raises(ValueError, match='one finite value per pulse'):
+        former.rereference(coef[:-1])

@@ -8,4 +8,4 @@ __version__ = '0.1.0'
 from .api import (form_image, ImageFormer, available_backends, BACKENDS, final_weights)  # noqa: F401
 from .exact import ExactFormer, plane_points, backproject  # noqa: F401
 from .memory import MemoryWarning  # noqa: F401
-from . import sim, plan  # noqa: F401
+from . import sim, plan, autofocus  # noqa: F401

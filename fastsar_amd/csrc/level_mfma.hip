@@ -10,13 +10,18 @@
 //   * the row's window is staged into LDS as polyphase planes of packed bfloat16 (Re, Im) words, hi and lo (the
 //     contraction runs over the taps in the order ph NWp + u for tap Dk u + ph, so a lane reads its contraction
 //     indices as adjacent word pairs and a half-wave reads consecutive words);
-//   * the children's complex taps for the row are built once, cooperatively, into LDS in the operand layout of
-//     v_mfma_f32_16x16x32_bf16 (each lane then reads its operand as two 16-byte words);
+//   * the children's complex taps for the row are built once into LDS in the operand layout of
+//     v_mfma_f32_16x16x32_bf16 (each lane then reads its operand as two 16-byte words, hi and lo in separate
+//     64-lane planes so that a wave's reads are conflict-free): a thread takes a run of taps of one child and one
+//     phase, r = Dk u + ph for u in [u0, u1), and steps exp(2 pi i sl r) along it by the factor exp(2 pi i sl Dk),
+//     two sine-cosine pairs per run instead of one per tap;
 //   * each wave multiplies its 16 outputs by the 8 children (16 columns: Re and Im of each child), rotates the
 //     result by rho and adds it into the NWP sliding accumulators of the pulse filter, which hold its 4 values per
 //     lane for the NWP outputs the row feeds; a finished output is stored.
-// The next row's planes and taps are built into the other half of a double buffer while the current row computes;
-// one barrier per row. Blocks are renumbered so that the blocks sharing a parent's rows run on one XCD.
+//   * each output's rotation rho_c(p, j) starts from a per-row table of its lane group's first value and the step
+//     between adjacent outputs, built with the taps.
+// The next row's planes, taps and rotations are built into the other half of a double buffer while the current row
+// computes; one barrier per row. Blocks are renumbered so that the blocks sharing a parent's rows run on one XCD.
 #include "fsh_common.h"
 #include "fsh_kernels.h"
 
@@ -68,20 +73,20 @@ __device__ __forceinline__ void split2(float a, float b, unsigned int& hi, unsig
     lo = __builtin_bit_cast(unsigned int, l);
 }
 
-constexpr int MAXG = 4;                    // tap entries built per thread and row (at most)
+constexpr int MAXG = 2;                    // tap runs built per thread and row (at most)
 constexpr int WAVES_MULTI = 8;             // waves of a block whose waves take several tiles
-
-struct TapK {                              // contraction index kappa: its tap value and tap number
-    float h, r;
-};
+constexpr int TAB_MAX = 128;               // (phase, u) tap positions: Dk NWp <= 4 ceil(Dk / 4) NWp = 8 (2 KS) <= 128
 
 }  // namespace
 
+// Four waves a SIMD (at most 128 registers) where the pulse filter's accumulators leave room: with a barrier per row,
+// the three waves a SIMD of the unconstrained build left the 12288 x 8192 image's level 2 at 131 ms against 123 ms.
 template <int KS, int NWP, int TPW>
-__global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI) void k_level_mfma(
+__global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1 && NWP <= 10 ? 4 : 1) void k_level_mfma(
     const float2* __restrict__ x, int P, int K, const float* __restrict__ c0, const float* __restrict__ sl, int C,
     const float* __restrict__ hk, int Lk, int Dk, int plk, int Ko, int NWp, int NPH, int PS, int JT, int njr,
-    const float* __restrict__ hp_t, int Dp, int plp, int Lp, int Po, int IS, int nseg, float kcf, float2* __restrict__ z)
+    const float* __restrict__ hp_t, int Dp, int plp, int Lp, int Po, int IS, int nseg, float kcf, int npc,
+    float2* __restrict__ z)
 {
     constexpr int MAXI = TPW == 1 ? 4 : 8;             // window words staged per thread and row (at most)
     const int NWV = blockDim.x >> 6;                   // waves: wave w takes tiles w, w + NWV, ... of the block's JT
@@ -105,25 +110,36 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI) void k_l
     const int jbase = 16 * JT * jr;                      // first output of this block's j-range
     const int WM = 16 * JT + NWp;                        // plane words used
 
-    // LDS: planes [2][2 (hi, lo)][NPH][PS] words, tap fragments [2][KS][64] x (hi, lo) uint4, contraction table
+    // LDS: planes [2][2 (hi, lo)][NPH][PS] words, tap fragments [2][KS][2 (hi, lo)][64] uint4, the A operand's pair
+    // offsets [8 KS], the taps' fragment words and values [TAB_MAX] each
     unsigned int* planes = lds_u;
     uint4_t* frag = reinterpret_cast<uint4_t*>(planes + 4 * NPH * PS);
-    TapK* tabk = reinterpret_cast<TapK*>(frag + 2 * KS * 64 * 2);
-    int* taba = reinterpret_cast<int*>(tabk + 16 * KS);  // [8 KS] pair offsets ph PS + u
+    int* taba = reinterpret_cast<int*>(frag + 2 * KS * 2 * 64);  // [8 KS] pair offsets ph PS + u
+    int* tabw = taba + 8 * KS;                                   // [Dk NWp] word of tap (ph, u), child 0, Re column
+    float* tabh = reinterpret_cast<float*>(tabw + TAB_MAX);     // [Dk NWp] its real tap value
+    const int RW = 4 * JT + 1;
+    float2* rotw = reinterpret_cast<float2*>(tabh + TAB_MAX);   // [2][CB][RW]: rho at each lane group's first output, step
 
     // contraction slot sigma = 16 s + 4 g + ew (k-step s, lane group g, word ew = 2 pr + e of the lane's operand)
     // holds tap (ph, u) = (4 pb + g, 2 up + e), where pr-pair column 2 s + pr = pb (NWp / 2) + up: the four lane groups
     // of a k-step read four consecutive planes at the same word, and planes are PS = 16 mod 64 words apart, so their
     // 16-word runs fall on disjoint banks
-    for (int i = tid; i < 16 * KS; i += nthr) {
-        const int st = i >> 4, g = (i >> 2) & 3, pr = (i >> 1) & 1, e = i & 1;
+    for (int i = tid; i < 8 * KS; i += nthr) {
+        const int st = i >> 3, g = (i >> 1) & 3, pr = i & 1;
         const int col = 2 * st + pr, pb = col / (NWp >> 1), up = col - pb * (NWp >> 1);
-        const int ph = 4 * pb + g, u = 2 * up + e, r = Dk * u + ph;
-        TapK t;
-        t.h = (ph < Dk && r < Lk) ? hk[r] : 0.f;
-        t.r = (float)r;
-        tabk[i] = t;
-        if (e == 0) taba[i >> 1] = ph * PS + u;
+        taba[i] = (4 * pb + g) * PS + 2 * up;
+    }
+    // tap (ph, u), ph < Dk, u < NWp: its word in a buffer's hi plane at lane 16 g (child 0, Re column) and its value;
+    // positions past the filter (r >= Lk) keep the zeros written below
+    for (int i = tid; i < Dk * NWp; i += nthr) {
+        const int ph = i / NWp, u = i - ph * NWp, r = Dk * u + ph;
+        const int g = ph & 3, col = (ph >> 2) * (NWp >> 1) + (u >> 1), st = col >> 1, ew = 2 * (col & 1) + (u & 1);
+        tabw[i] = r < Lk ? ((2 * st) * 64 + 16 * g) * 4 + ew : -1;
+        tabh[i] = r < Lk ? hk[r] : 0.f;
+    }
+    {
+        unsigned int* fw = reinterpret_cast<unsigned int*>(frag);
+        for (int i = tid; i < 2 * KS * 2 * 64 * 4; i += nthr) fw[i] = 0u;
     }
     __syncthreads();
 
@@ -136,7 +152,7 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI) void k_l
     int it_lds[MAXI], it_k[MAXI];
     #pragma unroll
     for (int k = 0; k < MAXI; ++k) {
-        const int idx = tid + k * nthr;
+        const int idx = (nthr - 1 - tid) + k * nthr;       // from the last thread down: tap pieces fill the first
         const int ph = idx / WM, m = idx - ph * WM;
         const int kk = Dk * (jbase + m) + ph - plk;
         const bool ok = idx < Dk * WM;                 // the planes past Dk hold zeros, written once below
@@ -147,27 +163,52 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI) void k_l
         #pragma unroll
         for (int hb = 0; hb < 4; ++hb) planes[(size_t)hb * NPH * PS + Dk * PS + i] = 0u;
     }
-    const int gc = tid & 7, gcg = cb * CB + gc;
-    float g_h[MAXG], g_r[MAXG];
-    int g_w0[MAXG], g_w1[MAXG];
+    // tap runs w = tid + k nthr of the CB Dk (child, phase) runs cut into npc pieces: run q = ph CB + c, piece
+    // w mod npc takes u in [u0, u1) (the u with r = Dk u + ph < Lk); children past C keep their zero taps
+    const int ulen = (NWp + npc - 1) / npc;
+    int g_c[MAXG], g_ph[MAXG], g_u0[MAXG], g_u1[MAXG];
     #pragma unroll
     for (int k = 0; k < MAXG; ++k) {
-        const int i = tid + k * nthr, kap = i >> 3;
-        const bool ok = kap < 16 * KS;
-        const TapK e = tabk[ok ? kap : 0];
-        g_h[k] = (ok && gcg < C) ? e.h : 0.f;
-        g_r[k] = e.r;
-        const int s = kap >> 4, g = (kap >> 2) & 3, ew = kap & 3, L0 = 16 * g + 2 * gc;
-        g_w0[k] = ok ? ((s * 64 + L0) * 2) * 4 + ew : -1;      // word of the hi uint4; +4: the lo one
-        g_w1[k] = ok ? ((s * 64 + L0 + 1) * 2) * 4 + ew : -1;
+        const int w = tid + k * nthr, q = w / npc, pc = w - q * npc;
+        const int c = q % CB, ph = q / CB;
+        const int nu = ph < Dk ? (Lk - ph + Dk - 1) / Dk : 0;       // taps of this phase inside the filter
+        const bool ok = q < CB * Dk && cb * CB + c < C;
+        g_c[k] = ok ? c : -1;
+        g_ph[k] = ph;
+        g_u0[k] = pc * ulen;
+        g_u1[k] = min(nu, pc * ulen + ulen);
+    }
+    // rotation items i = tid + k nthr of the CB x RW table: child c = i / RW; s = i mod RW < 4 JT: rho_c(p, j) at the
+    // first output j = 16 (jr JT + s / 4) + 4 (s mod 4) of a lane group, exp(2 pi i (c0 + (Dk j - plk - kc) sl));
+    // s = 4 JT: the step between adjacent outputs, exp(2 pi i Dk sl)
+    constexpr int MAXR = TPW == 1 ? 1 : 2;
+    int r_i[MAXR];
+    float r_m[MAXR], r_a[MAXR], r_b[MAXR];
+    #pragma unroll
+    for (int k = 0; k < MAXR; ++k) {
+        const int i = tid + k * nthr, c = i / RW, si = i - c * RW;
+        r_i[k] = i < CB * RW ? i : -1;
+        r_m[k] = si < 4 * JT ? (float)(Dk * (16 * (jr * JT + (si >> 2)) + 4 * (si & 3)) - plk) - kcf : (float)Dk;
+        r_a[k] = r_b[k] = 0.f;
     }
     float2 xv[MAXI];
-    float gsl = 0.f;
+    float gsl[MAXG];
     auto load_row = [&](int p) {
         const float2* xr = xn + (size_t)p * K;
         #pragma unroll
         for (int k = 0; k < MAXI; ++k) xv[k] = it_k[k] >= 0 ? xr[it_k[k]] : make_float2(0.f, 0.f);
-        gsl = gcg < C ? sl[((size_t)n * C + gcg) * P + p] : 0.f;
+        #pragma unroll
+        for (int k = 0; k < MAXG; ++k)
+            gsl[k] = (g_c[k] >= 0 && g_u0[k] < g_u1[k]) ? sl[((size_t)n * C + cb * CB + g_c[k]) * P + p] : 0.f;
+        #pragma unroll
+        for (int k = 0; k < MAXR; ++k) {
+            const int c = r_i[k] / RW, cgl = cb * CB + c;
+            if (r_i[k] >= 0 && cgl < C) {
+                const size_t ci = ((size_t)n * C + cgl) * P + p;
+                r_a[k] = r_i[k] - c * RW < 4 * JT ? c0[ci] : 0.f;
+                r_b[k] = sl[ci];
+            }
+        }
     };
     auto store_row = [&](int b) {
         unsigned int* XH = planes + (2 * b) * NPH * PS;
@@ -181,20 +222,40 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI) void k_l
                 XL[it_lds[k]] = lo;
             }
         }
-        unsigned int* FH = reinterpret_cast<unsigned int*>(frag + (size_t)b * KS * 64 * 2);
+        unsigned int* FH = reinterpret_cast<unsigned int*>(frag + (size_t)b * KS * 2 * 64);
         #pragma unroll
         for (int k = 0; k < MAXG; ++k) {
-            if (g_w0[k] >= 0) {
+            if (g_c[k] >= 0 && g_u0[k] < g_u1[k]) {
+                // exp(2 pi i sl r) at r = Dk u0 + ph, then times exp(2 pi i sl Dk) per step of u
+                float cs, sn, dc, ds;
+                cis_lin(0.f, gsl[k], (float)(Dk * g_u0[k] + g_ph[k]), cs, sn);
+                cis_lin(0.f, gsl[k], (float)Dk, dc, ds);
+                const int* tw = tabw + g_ph[k] * NWp;
+                const float* th = tabh + g_ph[k] * NWp;
+                const int cw = 8 * g_c[k];                     // lane 2 c: the child's Re column
+                #pragma unroll 1
+                for (int u = g_u0[k]; u < g_u1[k]; ++u) {
+                    const float h = th[u];
+                    const int wd = tw[u] + cw;
+                    unsigned int hi, lo;
+                    split2(h * cs, -(h * sn), hi, lo);         // column Re y_c: (Re g, -Im g)
+                    FH[wd] = hi;
+                    FH[wd + 256] = lo;
+                    // column Im y_c (lane 2 c + 1): (Im g, Re g), the same halves swapped with the sign of Im restored
+                    FH[wd + 4] = __builtin_amdgcn_alignbit(hi, hi, 16) ^ 0x8000u;
+                    FH[wd + 260] = __builtin_amdgcn_alignbit(lo, lo, 16) ^ 0x8000u;
+                    const float nc = fmaf(cs, dc, -(sn * ds));
+                    sn = fmaf(cs, ds, sn * dc);
+                    cs = nc;
+                }
+            }
+        }
+        #pragma unroll
+        for (int k = 0; k < MAXR; ++k) {
+            if (r_i[k] >= 0) {
                 float cs, sn;
-                cis_lin(0.f, gsl, g_r[k], cs, sn);
-                const float gr = g_h[k] * cs, gi = g_h[k] * sn;
-                unsigned int hi, lo;
-                split2(gr, -gi, hi, lo);                       // column Re y_c: (Re g, -Im g)
-                FH[g_w0[k]] = hi;
-                FH[g_w0[k] + 4] = lo;
-                // column Im y_c: (Im g, Re g), the same halves swapped with the sign of Im restored
-                FH[g_w1[k]] = __builtin_amdgcn_alignbit(hi, hi, 16) ^ 0x8000u;
-                FH[g_w1[k] + 4] = __builtin_amdgcn_alignbit(lo, lo, 16) ^ 0x8000u;
+                cis_lin(r_a[k], r_b[k], r_m[k], cs, sn);
+                rotw[b * CB * RW + r_i[k]] = make_float2(cs, sn);
             }
         }
     };
@@ -251,37 +312,26 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI) void k_l
     // rows before 0 (first segment): zero rows, only the period bookkeeping
     for (int p = pbeg; p < plo; ++p)
         if (++pph == Dp) { end_period(); pph = 0; }
-    // the rotation constants of the row being computed (c0, slope of this lane's child), loaded a row ahead
-    float rc0 = 0.f, rsl = 0.f, nc0 = 0.f, nsl = 0.f;
-    auto load_rot = [&](int p, float& a, float& b) {
-        if (cok) {
-            a = c0[((size_t)n * C + cg) * P + p];
-            b = sl[((size_t)n * C + cg) * P + p];
-        }
-    };
     if (plo < phi) {
         load_row(plo);
-        load_rot(plo, rc0, rsl);
         store_row(0);
     }
     __syncthreads();
     for (int p = plo; p < phi; ++p) {
         const int b = (p - plo) & 1;
         const bool more = p + 1 < phi;
-        if (more) {
-            load_row(p + 1);
-            load_rot(p + 1, nc0, nsl);
-        }
+        if (more) load_row(p + 1);
         const float* hp = hp_t + pph * NWP;              // the row's pulse taps (uniform)
-        float drr = 0.f, dri = 0.f;
-        if (tact[0]) cis_lin(0.f, rsl, (float)Dk, drr, dri);
+        const float2* rl = rotw + (b * CB + c_l) * RW;   // this lane's child's rotations for the row
+        const float2 dr = rl[4 * JT];
+        const float drr = dr.x, dri = dr.y;
         #pragma unroll
         for (int kt = 0; kt < TPW; ++kt) {
           if (tact[kt]) {
             const int jl = 16 * (wv + kt * NWV) + lc;      // A row (output) of this lane within the block's window
             const unsigned int* XH = planes + (2 * b) * NPH * PS;
             const unsigned int* XL = XH + NPH * PS;
-            const uint4_t* F = frag + (size_t)b * KS * 64 * 2;
+            const uint4_t* F = frag + (size_t)b * KS * 2 * 64;
             floatx4_t acc = {};
             #pragma unroll
             for (int s = 0; s < KS; ++s) {
@@ -292,7 +342,7 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI) void k_l
                 uint4_t wh, wl;
                 wh.x = a0[0]; wh.y = a0[1]; wh.z = a1[0]; wh.w = a1[1];
                 wl.x = b0[0]; wl.y = b0[1]; wl.z = b1[0]; wl.w = b1[1];
-                const uint4_t gh = F[(s * 64 + lane) * 2], gl = F[(s * 64 + lane) * 2 + 1];
+                const uint4_t gh = F[(2 * s) * 64 + lane], gl = F[(2 * s + 1) * 64 + lane];
                 const bf16x8_t Ah = __builtin_bit_cast(bf16x8_t, wh), Al = __builtin_bit_cast(bf16x8_t, wl);
                 const bf16x8_t Bh = __builtin_bit_cast(bf16x8_t, gh), Bl = __builtin_bit_cast(bf16x8_t, gl);
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bh, acc, 0, 0, 0);
@@ -301,8 +351,8 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI) void k_l
             }
             // rho_c(p, j) at the lane's outputs j0 .. j0 + 3: the lane pair (t = 0, 1) holds (Re, Im); output
             // m - NWP + 1 + s of the pulse filter takes tap Dp (NWP - 1 - s) + ph of this row
-            float er, ei;
-            cis_lin(rc0, rsl, (float)(Dk * (jout0 + 16 * kt * NWV) - plk) - kcf, er, ei);
+            const float2 e0 = rl[4 * (wv + kt * NWV) + grp];
+            float er = e0.x, ei = e0.y;
             #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const float mine = acc[i];
@@ -318,13 +368,25 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI) void k_l
         }
         if (++pph == Dp) { end_period(); pph = 0; }
         if (more) store_row(b ^ 1);
-        rc0 = nc0;
-        rsl = nsl;
         __syncthreads();
     }
     // rows past the last read: finish the open period, then every period until the segment's last output is stored
     if (pph != 0) { end_period(); pph = 0; }
     while (pm - NWP + 1 < i1) end_period();
+}
+
+// pieces each (child, phase) run of taps is cut into: runs of about 3 taps (two sine-cosine pairs per run against a
+// multiply per tap; on the 12288 x 8192 image's level 2, runs of 2-3 taps over all threads took 116.5 ms, of 5 taps
+// over half the threads 119.0 ms, of 10 taps 138.2 ms, one pair per tap 131 ms), fewer pieces where the block has
+// fewer threads than pieces
+static int tap_pieces(int Dk, int NWp, int nthr)
+{
+    int len = 3;
+    if (const char* e = getenv("FSH_LMFMA_TAPRUN")) len = std::max(1, atoi(e));    // sweeps
+    const int runs = CB * Dk;
+    int npc = std::max(1, (NWp + len - 1) / len);
+    while (npc > 1 && runs * npc > nthr) --npc;
+    return npc;
 }
 
 // the launch shape, or false where the form does not apply: both axes decimated, the range taps in 128 contraction
@@ -367,11 +429,14 @@ bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int
     }
     PS = 16 * JT + NWp;
     PS = ((PS - 16 + 63) / 64) * 64 + 16;                  // 16 mod 64 words: the lane groups' runs on disjoint banks
-    lds = (size_t)4 * 4 * NPH * PS + (size_t)2 * KS * 64 * 2 * 16 + (size_t)16 * KS * sizeof(TapK) + (size_t)8 * KS * 4;
+    lds = (size_t)4 * 4 * NPH * PS + (size_t)2 * KS * 64 * 2 * 16 + (size_t)8 * KS * 4 + (size_t)2 * TAB_MAX * 4 +
+          (size_t)2 * CB * (4 * JT + 1) * 8;
     if (lds > 80 * 1024) return false;
     // the per-thread staging registers hold a row's window words and tap entries
     const int nthr = 64 * nwv, maxi = tpw == 1 ? 4 : 8;
-    if (Dk * (16 * JT + NWp) > maxi * nthr || 16 * KS * CB > MAXG * nthr) return false;
+    if (Dk * (16 * JT + NWp) > maxi * nthr || Dk * NWp > TAB_MAX || CB * Dk * tap_pieces(Dk, NWp, nthr) > MAXG * nthr ||
+        CB * (4 * JT + 1) > (tpw == 1 ? 1 : 2) * nthr)
+        return false;
     // segments of output pulses: enough blocks to fill the device, a halo below a fifth of the rows
     const long blocks0 = (long)Np * ((C + CB - 1) / CB) * njr;
     nseg = 1;
@@ -413,8 +478,9 @@ int launch_level_mfma(const float2* x, int Np, int P, int K, const float* c0, co
     if (G > 2147483647L) { set_error("level_mfma: too many blocks"); return E_INVALID; }
     const dim3 grid((unsigned)G), block((unsigned)(64 * nwv));
     const float kcf = (float)kc;
+    const int npc = tap_pieces(Dk, NWp, 64 * nwv);
 #define FSH_LM(A, B, T) hipLaunchKernelGGL((k_level_mfma<A, B, T>), grid, block, lds, s, x, P, K, c0, sl, C, hk, Lk, Dk, plk, \
-                                           Ko, NWp, NPH, PS, JT, njr, hp_t, Dp, plp, Lp, Po, IS, nseg, kcf, z)
+                                           Ko, NWp, NPH, PS, JT, njr, hp_t, Dp, plp, Lp, Po, IS, nseg, kcf, npc, z)
 #define FSH_LM_T(A, B)                                                                                                \
     switch (tpw) {                                                                                                    \
         case 1: FSH_LM(A, B, 1); break; case 2: FSH_LM(A, B, 2); break; default: FSH_LM(A, B, 3); break;             \

@@ -452,7 +452,9 @@ __global__ __launch_bounds__(256) void k_fir_p(const float2* __restrict__ y, int
 template <int NW, bool BATCH>
 __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__ y, int B, int P, int Ko,
                                                        const float* __restrict__ taps, int L, int pl, int D, int Po, int IC,
-                                                       float2* __restrict__ z, const float* __restrict__ hs)
+                                                       float2* __restrict__ z, const float* __restrict__ hs,
+                                                       const float* __restrict__ rc0, const float* __restrict__ rsl, int rD,
+                                                       int rpl, double rkc)
 {
     typedef _Float16 h2 __attribute__((ext_vector_type(2)));
     const float hinv = hs ? hs[1] : 1.f;             // float16 input: its scale undone by hs[1]
@@ -469,6 +471,10 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
     const float2* yb = y + (size_t)b * P * Ko + j;
     const h2* ybh = yh + (size_t)b * P * Ko + j;
     float2* zb = z + (size_t)b * Po * Ko + j;
+    // a deferred range rotation: sample (p, j) of row b times exp(2 pi i (c0 + (rD j - rpl - rkc) sl)) of row p
+    const double rj = (double)(rD * j - rpl) - rkc;
+    const float* rc0b = rc0 ? rc0 + (size_t)b * P : nullptr;
+    const float* rslb = rsl ? rsl + (size_t)b * P : nullptr;
     float2 acc[NW];
     #pragma unroll
     for (int s = 0; s < NW; ++s) acc[s] = make_float2(0.f, 0.f);
@@ -486,6 +492,12 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
                         v = make_float2((float)w.x * hinv, (float)w.y * hinv);
                     } else {
                         v = yb[(size_t)p * Ko];
+                    }
+                    if (rc0b) {
+                        // the phase in cycles wrapped in float64, then the hardware sine and cosine (revolutions)
+                        const double c = fma(rj, (double)rslb[p], (double)rc0b[p]);
+                        const float w = (float)(c - rint(c));
+                        v = cmul(v, make_float2(__builtin_amdgcn_cosf(w), __builtin_amdgcn_sinf(w)));
                     }
                 }
                 #pragma unroll
@@ -979,8 +991,9 @@ int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, c
 }
 
 int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
-                     int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s)
+                     int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s, bool* deferred)
 {
+    if (deferred) *deferred = false;
     if (Np < 1 || P < 1 || K < 1 || Cn < 1 || L < 1 || D < 1 || Ko < 1 || pl < 0) {
         set_error("rot_fir_k: invalid shape");
         return E_INVALID;
@@ -997,7 +1010,10 @@ int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, con
         int st = E_INVALID;
         // the row form amortizes each row's taps over its M-tiles: it wins from four M-tiles (128 outputs) up
         const bool row_form = force && strcmp(force, "mfma") == 0 ? true : (Ko + 31) / 32 >= 4;
-        if (!want_mfma1 && row_form) st = launch_rot_fir_k_mf2(x, Np, P, K, c0, sl, Cn, taps, L, pl, D, Ko, kc, y, s);
+        if (!want_mfma1 && row_form) {
+            st = launch_rot_fir_k_mf2(x, Np, P, K, c0, sl, Cn, taps, L, pl, D, Ko, kc, y, s, nullptr, deferred == nullptr);
+            if (st == OK && deferred) *deferred = true;
+        }
         if (st == E_INVALID) st = launch_rot_fir_k_mfma(x, Np, P, K, c0, sl, Cn, taps, L, pl, D, Ko, kc, y, s);
         if (st != E_INVALID) return st;
     }
@@ -1030,8 +1046,14 @@ int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, con
     return OK;
 }
 
+bool fir_p_streams(int L, int D)
+{
+    const char* force = getenv("FSH_FIRP_KERNEL");
+    return D >= 1 && (L + D - 1) / D <= 16 && !force;
+}
+
 int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float2* z, hipStream_t s,
-                 const float* hs)
+                 const float* hs, const DeferredRot* rot)
 {
     if (B < 1 || P < 1 || Ko < 1 || L < 1 || D < 1 || Po < 1 || pl < 0) {
         set_error("fir_p: invalid shape");
@@ -1043,15 +1065,25 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
         set_error("fir_p: a float16 input is read by the streaming kernel only (at most 16 taps per phase)");
         return E_INVALID;
     }
+    if (rot && !fir_p_streams(L, D)) {
+        set_error("fir_p: a deferred range rotation is applied by the streaming kernel only");
+        return E_INVALID;
+    }
     if (NW <= 16 && !(force && strcmp(force, "lds") == 0)) {
         const int IC = std::min(Po, 128);
         dim3 grid((unsigned)(((long)B * Ko + 255) / 256), (Po + IC - 1) / IC);
         const bool batch = force && strcmp(force, "stream4") == 0;
+        const float* rc0 = rot ? rot->c0 : nullptr;
+        const float* rsl = rot ? rot->sl : nullptr;
+        const int rD = rot ? rot->D : 0, rpl = rot ? rot->pl : 0;
+        const double rkc = rot ? rot->kc : 0.0;
         const size_t lds = batch ? 0 : (size_t)4 * D * NW;
 #define FSH_FP(NN)                                                                                                       \
     do {                                                                                                                 \
-        if (batch) hipLaunchKernelGGL((k_fir_p_stream<NN, true>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs); \
-        else hipLaunchKernelGGL((k_fir_p_stream<NN, false>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs);     \
+        if (batch) hipLaunchKernelGGL((k_fir_p_stream<NN, true>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs, \
+                                      rc0, rsl, rD, rpl, rkc);                                                           \
+        else hipLaunchKernelGGL((k_fir_p_stream<NN, false>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs,   \
+                                rc0, rsl, rD, rpl, rkc);                                                                 \
     } while (0)
         switch (NW) {
             case 1: FSH_FP(1); break;

@@ -317,22 +317,33 @@ struct Former {
             // the low-precision form: the range-filtered rows of a decimated-in-pulse level in float16 where the row
             // form of the matrix-core filter takes the level and the streaming pulse filter reads it back
             const float* hs = nullptr;
+            // the range filter's rho left to the streaming pulse filter, which applies it per sample as it reads y
+            // (FSH_DEFER_ROT=0 keeps it in the range filter)
+            const char* dr_env = getenv("FSH_DEFER_ROT");
+            const bool can_defer = h.Dp > 1 && fir_p_streams(h.Lp, h.Dp) && !(dr_env && strcmp(dr_env, "0") == 0);
+            bool deferred = false;
             if (half_mid && h.Dk > 1 && h.Dp > 1 && (h.Lp + h.Dp - 1) / h.Dp <= 16 && !getenv("FSH_FIRP_KERNEL")) {
                 st = launch_rot_fir_k_mf2(cur, (int)parents, h.P, h.K, c0, sl, ch, L.taps_k, h.Lk, h.plk, h.Dk, h.Ko, h.kc,
-                                          buf[yb], stream, hsc + 2 * l);
-                if (st == OK) hs = hsc + 2 * l;
-                else if (st != E_INVALID) return st;
+                                          buf[yb], stream, hsc + 2 * l, !can_defer);
+                if (st == OK) {
+                    hs = hsc + 2 * l;
+                    deferred = can_defer;
+                } else if (st != E_INVALID) {
+                    return st;
+                }
             }
             if (!hs && (st = launch_rot_fir_k(cur, (int)parents, h.P, h.K, c0, sl, ch, h.Dk > 1 ? L.taps_k : one,
-                                              h.Dk > 1 ? h.Lk : 1, h.Dk > 1 ? h.plk : 0, h.Dk, h.Ko, h.kc, buf[yb], stream)))
+                                              h.Dk > 1 ? h.Lk : 1, h.Dk > 1 ? h.plk : 0, h.Dk, h.Ko, h.kc, buf[yb], stream,
+                                              can_defer ? &deferred : nullptr)))
                 return st;
+            const DeferredRot drot{c0, sl, h.Dk, h.Dk > 1 ? h.plk : 0, h.kc};
             mark_end(ST_ROT);
             cur = buf[yb];
             curbuf = yb;
             if (h.Dp > 1) {
                 mark_begin();
                 if ((st = launch_fir_p(buf[yb], (int)(parents * ch), h.P, h.Ko, L.taps_p, h.Lp, h.plp, h.Dp, h.Po, buf[1 - yb],
-                                       stream, hs)))
+                                       stream, hs, deferred ? &drot : nullptr)))
                     return st;
                 mark_end(ST_FIRP);
                 cur = buf[1 - yb];

@@ -53,7 +53,8 @@ struct PairTab {
 
 }  // namespace
 
-template <int KS>
+// ROT false: y holds the filtered row without rho (the streaming pulse filter applies it as it reads y)
+template <int KS, bool ROT>
 __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restrict__ x, int P, int K, const float* __restrict__ c0,
                                                         const float* __restrict__ sl, int Cn, const float* __restrict__ taps,
                                                         int L, int pl, int D, int Ko, double kc, int NWp, int NPH, int PS,
@@ -146,9 +147,11 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
                 }
             }
         }
-        float sDr, sDi, s8r, s8i;
-        cis_cyc(b * (double)D, sDr, sDi);
-        cis_cyc(b * (double)(8 * D), s8r, s8i);
+        float sDr = 1.f, sDi = 0.f, s8r = 1.f, s8i = 0.f;
+        if (ROT) {
+            cis_cyc(b * (double)D, sDr, sDi);
+            cis_cyc(b * (double)(8 * D), s8r, s8i);
+        }
         for (int mt = m0 + ws; mt < m1; mt += wpn) {
             floatx16_t acc = {};
             const int jb = (mt - m0) * 32 + col;
@@ -168,10 +171,14 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
             }
             // rho at rows 4q, 4q + 1, ... of this tile: seeded at row 4q, stepped by D samples, 8 rows apart by 8 D;
             // the lane pair (t = 0, 1) holds (Re, Im) of one output: Re' = Re er - Im ei, Im' = Im er + Re ei
-            float rr, ri;
-            cis_cyc(fma((double)(D * (mt * 32 + 4 * q) - pl) - kc, b, a), rr, ri);
+            if (!ROT) {
+                #pragma unroll
+                for (int i = 0; i < 16; ++i) stg[(col >> 1) * 66 + 2 * (8 * (i >> 2) + 4 * q + (i & 3)) + t] = acc[i];
+            }
+            float rr = 1.f, ri = 0.f;
+            if (ROT) cis_cyc(fma((double)(D * (mt * 32 + 4 * q) - pl) - kc, b, a), rr, ri);
             #pragma unroll
-            for (int g = 0; g < 4; ++g) {
+            for (int g = 0; g < (ROT ? 4 : 0); ++g) {
                 float er = rr, ei = ri;
                 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
@@ -234,7 +241,7 @@ static bool mf2_shape(int L, int D, int Ko, int& NWp, int& KS, int& NPH, int& PS
 }
 
 int launch_rot_fir_k_mf2(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
-                         int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s, const float* hs)
+                         int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s, const float* hs, bool rot)
 {
     int NWp, KS, NPH, PS, MC, chunks;
     size_t lds;
@@ -248,8 +255,13 @@ int launch_rot_fir_k_mf2(const float2* x, int Np, int P, int K, const float* c0,
         // a float16 output counts 4 bytes a sample: the chunk's offset in float2 units is halved
         float2* ys = hs ? reinterpret_cast<float2*>(reinterpret_cast<char*>(y) + (size_t)n0 * Cn * P * Ko * 4)
                         : y + (size_t)n0 * Cn * P * Ko;
-#define FSH_M2(KK) hipLaunchKernelGGL(k_rot_fir_k_mf2<KK>, grid, dim3(256), lds, s, xs, P, K, c0s, sls, Cn, taps, L, pl, D, Ko, \
-                                      kc, NWp, NPH, PS, MC, ys, hs)
+#define FSH_M2(KK)                                                                                                      \
+    do {                                                                                                                \
+        if (rot) hipLaunchKernelGGL((k_rot_fir_k_mf2<KK, true>), grid, dim3(256), lds, s, xs, P, K, c0s, sls, Cn, taps, L, pl, \
+                                    D, Ko, kc, NWp, NPH, PS, MC, ys, hs);                                               \
+        else hipLaunchKernelGGL((k_rot_fir_k_mf2<KK, false>), grid, dim3(256), lds, s, xs, P, K, c0s, sls, Cn, taps, L, pl, \
+                                D, Ko, kc, NWp, NPH, PS, MC, ys, hs);                                                   \
+    } while (0)
         switch (KS) {
             case 1: FSH_M2(1); break;   case 2: FSH_M2(2); break;   case 3: FSH_M2(3); break;   case 4: FSH_M2(4); break;
             case 5: FSH_M2(5); break;   case 6: FSH_M2(6); break;   case 7: FSH_M2(7); break;   case 8: FSH_M2(8); break;

@@ -60,13 +60,15 @@ __device__ __forceinline__ float2 load_sample(const void* s, size_t idx, const f
 template <class TOUT, int DT = 0>
 __global__ __launch_bounds__(256) void k_pad_window(const void* __restrict__ s, const float* __restrict__ wp,
                                                      const float* __restrict__ wk, int n, int K, int nfft, TOUT* __restrict__ pad,
-                                                     int* __restrict__ bad, int p0, const float* __restrict__ inv = nullptr)
+                                                     int* __restrict__ bad, int p0, const float* __restrict__ inv = nullptr,
+                                                     const double2* __restrict__ ramp = nullptr)
 {
     const size_t total = (size_t)n * K;
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
         const int p = (int)(idx / K), k = (int)(idx - (size_t)p * K), h = K / 2;
-        const float2 v = load_sample<DT>(s, idx, inv, p0 + p);
+        float2 v = load_sample<DT>(s, idx, inv, p0 + p);
         if (bad && !(isfinite(v.x) && isfinite(v.y))) atomicMin(bad, p0 + p);
+        v = apply_ramp(v, ramp, p0 + p, k);
         const double g = wp ? (double)wp[p] * (double)wk[k] : 1.0;
         TOUT o;
         if (sizeof(TOUT) == 8) {
@@ -322,6 +324,8 @@ struct Exact {
     void* winh = nullptr;            // [chunk, W] float16 profiles (half2, or two half2 per sample in pairs)
     unsigned* smax = nullptr;        // the chunk's largest profile magnitude (float bits)
     float* pinv = nullptr;           // a float16 history's per-pulse inverse scales (host input)
+    double* ramp = nullptr;          // [P] (a, b) cycles: the phase re-reference applied on the way in (fsh_exact_set_ramp)
+    bool use_ramp = false;
     float2* image = nullptr;
     int* bad = nullptr;
     hipfftHandle plan = 0;
@@ -333,6 +337,7 @@ struct Exact {
         if (have_plan) hipfftDestroy(plan);
         dev_free(ant); dev_free(ref); dev_free(cen); dev_free(blo); dev_free(wp); dev_free(wk); dev_free(hist);
         dev_free(pad); dev_free(prof); dev_free(win); dev_free(winh); dev_free(smax); dev_free(pinv); dev_free(image); dev_free(bad);
+        dev_free(ramp);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -464,16 +469,17 @@ static int exact_form_impl(Exact* e, const void* S, int dtype, const float* puls
         const dim3 gp((unsigned)std::min<size_t>((tot + 255) / 256, 1 << 16));
         const void* sp = src + (size_t)p0 * d.K * ssz;
         const float* wpp = e->wp ? e->wp + p0 : nullptr;
+        const double2* rmp = e->use_ramp ? reinterpret_cast<const double2*>(e->ramp) : nullptr;
         if (dtype == FSH_C32F16)
-            hipLaunchKernelGGL((k_pad_window<float2, 1>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv);
+            hipLaunchKernelGGL((k_pad_window<float2, 1>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv, rmp);
         else if (dtype == FSH_C32BF16)
-            hipLaunchKernelGGL((k_pad_window<float2, 2>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv);
+            hipLaunchKernelGGL((k_pad_window<float2, 2>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv, rmp);
         else if (dtype == FSH_CI16BE)
-            hipLaunchKernelGGL((k_pad_window<float2, 3>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv);
+            hipLaunchKernelGGL((k_pad_window<float2, 3>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv, rmp);
         else if (dtype == FSH_CI8)
-            hipLaunchKernelGGL((k_pad_window<float2, 4>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv);
+            hipLaunchKernelGGL((k_pad_window<float2, 4>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv, rmp);
         else
-            hipLaunchKernelGGL((k_pad_window<float2, 0>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv);
+            hipLaunchKernelGGL((k_pad_window<float2, 0>), gp, dim3(256), 0, e->stream, sp, wpp, e->wk, n, d.K, d.nfft, e->pad, e->bad, p0, inv, rmp);
         FSH_TRY(hipGetLastError());
         if ((st = fft_status(hipfftExecC2C(e->plan, (hipfftComplex*)e->pad, (hipfftComplex*)e->prof, HIPFFT_BACKWARD),
                              "hipfftExecC2C")))
@@ -517,6 +523,22 @@ int fsh_exact_form_typed(void* handle, const void* S, int dtype, const float* pu
     if (!handle || !S || !image) { set_error("exact_form_typed: null argument"); return E_INVALID; }
     if (check_history_type(dtype, pulse_scale)) return E_INVALID;
     return exact_form_impl(static_cast<Exact*>(handle), S, dtype, pulse_scale, S_on_device, image, image_on_device, bad_pulse);
+}
+
+int fsh_exact_set_ramp(void* handle, const double* ab)
+{
+    if (!handle) { set_error("exact_set_ramp: null handle"); return E_INVALID; }
+    Exact* e = static_cast<Exact*>(handle);
+    if (!ab) {
+        e->use_ramp = false;
+        return OK;
+    }
+    int st;
+    if (!e->ramp && (st = dev_alloc((void**)&e->ramp, (size_t)e->d.P * 2 * sizeof(double)))) return st;
+    FSH_TRY(hipMemcpyAsync(e->ramp, ab, (size_t)e->d.P * 2 * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    FSH_TRY(hipStreamSynchronize(e->stream));
+    e->use_ramp = true;
+    return OK;
 }
 
 int fsh_exact_destroy(void* handle)

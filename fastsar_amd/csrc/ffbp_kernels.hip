@@ -32,17 +32,6 @@ __device__ __forceinline__ void note_max(float m, unsigned* xmax)
     if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(xmax, __float_as_uint(m));
 }
 
-// the phase re-reference of a CPHD read (fastsar/io.py:384-420, fastsar/cphd.py:391-401) applied on the way in:
-// sample k of pulse p times exp(2 pi i (a_p + b_p k)), the phase in cycles formed in float64 and wrapped
-__device__ __forceinline__ float2 apply_ramp(float2 v, const double2* ramp, int p, int k)
-{
-    if (!ramp) return v;
-    const double2 ab = ramp[p];
-    float cs, sn;
-    cis_cyc(fma((double)k, ab.y, ab.x), cs, sn);
-    return make_float2(fmaf(v.x, cs, -(v.y * sn)), fmaf(v.x, sn, v.y * cs));
-}
-
 __global__ __launch_bounds__(256) void k_prepare(const float2* __restrict__ S, int P, int K, const float* __restrict__ wp,
                                                   const float* __restrict__ wk, float2* __restrict__ x, int* __restrict__ bad,
                                                   unsigned* __restrict__ xmax, const double2* __restrict__ ramp)

@@ -34,6 +34,17 @@ __device__ __forceinline__ void cis_cyc(double c, float& cs, float& sn)
     sincospif(2.0f * w, &sn, &cs);
 }
 
+// the phase re-reference of a CPHD read (fastsar/io.py:384-420, fastsar/cphd.py:391-401) applied on the way in:
+// sample k of pulse p times exp(2 pi i (a_p + b_p k)), the phase in cycles formed in float64 and wrapped
+__device__ __forceinline__ float2 apply_ramp(float2 v, const double2* ramp, int p, int k)
+{
+    if (!ramp) return v;
+    const double2 ab = ramp[p];
+    float cs, sn;
+    cis_cyc(fma((double)k, ab.y, ab.x), cs, sn);
+    return make_float2(fmaf(v.x, cs, -(v.y * sn)), fmaf(v.x, sn, v.y * cs));
+}
+
 // Orders one wave's LDS accesses across its lanes: writes before the barrier are seen by reads after it, and reads
 // before it complete before writes after it. A wave's LDS instructions execute in issue order, so this only keeps
 // the compiler from moving accesses across it (wavefront-scope fences); it does not wait for other waves.

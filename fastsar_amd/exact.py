@@ -46,8 +46,9 @@ def _bind():
         L.fsh_sim_points.argtypes = [vp, vp, ci, ci, ctypes.c_double, ctypes.c_double, vp, vp, ci, vp, vp]
         L.fsh_add_noise.argtypes = [vp, ctypes.c_size_t, ctypes.c_double, ctypes.c_ulonglong, vp]
         L.fsh_exact_form_typed.argtypes = [vp, vp, ci, vp, ci, vp, ci, _hip.P_i32]
-        for n in ('fsh_exact_create', 'fsh_exact_form', 'fsh_exact_form_typed', 'fsh_exact_destroy', 'fsh_bp_points_f64',
-                  'fsh_sim_points', 'fsh_add_noise'):
+        L.fsh_exact_set_ramp.argtypes = [vp, vp]
+        for n in ('fsh_exact_create', 'fsh_exact_form', 'fsh_exact_form_typed', 'fsh_exact_set_ramp', 'fsh_exact_destroy',
+                  'fsh_bp_points_f64', 'fsh_sim_points', 'fsh_add_noise'):
             getattr(L, n).restype = ci
         L._exact_bound = True
     return L
@@ -233,6 +234,22 @@ class ExactFormer:
             _hip.check(L.fsh_exact_create(ctypes.byref(d), ctypes.byref(h)), 'exact_create')
             self._h = h
         return self._h
+
+    def rereference(self, coef, sign=-1):
+        """Apply the phase re-reference of a CPHD read on the device as each call takes its history: sample k of
+        pulse p times exp(sign 2 pi i f_k coef[p]), f_k = fmin + k df (fastsar/io.py:384-420; fastsar/cphd.py:401 moves
+        the reference to the grid centre with coef = 2 dref / c, sign -1). coef None removes it."""
+        h = self._handle()
+        if coef is None:
+            _hip.check(_bind().fsh_exact_set_ramp(h, None), 'exact_set_ramp')
+            return
+        coef = np.asarray(coef, np.float64)
+        if coef.shape != (self.P,) or not np.all(np.isfinite(coef)):
+            raise ValueError(f'coef must hold one finite value per pulse ({self.P}), got shape {coef.shape}')
+        if sign not in (-1, 1):
+            raise ValueError('sign must be +1 or -1')
+        ab = np.ascontiguousarray(np.stack([sign * self.fmin * coef, sign * self.df * coef], 1))
+        _hip.check(_bind().fsh_exact_set_ramp(h, ab.ctypes.data_as(ctypes.c_void_p)), 'exact_set_ramp')
 
     def memory(self):
         """dict(backend, needed, available, full_speed, parts) in bytes. (fastsar/exact.py:736-761)"""

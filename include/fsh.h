@@ -218,6 +218,9 @@ int fsh_exact_form(void* handle, const void* S, int S_on_device, void* image, in
 /* A history in a given type (FSH_C64, FSH_C32F16 with pulse_scale, FSH_C32BF16), as fsh_ffbp_form_typed. */
 int fsh_exact_form_typed(void* handle, const void* S, int dtype, const float* pulse_scale, int S_on_device, void* image,
                          int image_on_device, int* bad_pulse);
+/* The phase re-reference of a CPHD read (fastsar/io.py:384-420, fastsar/cphd.py:401), as fsh_ffbp_set_ramp: ab [P, 2]
+ * float64 (host) in cycles, sample k of pulse p times exp(2 pi i (a_p + b_p k)) before the window; NULL removes it. */
+int fsh_exact_set_ramp(void* handle, const double* ab);
 int fsh_exact_destroy(void* handle);
 
 /* ------------------------------------------------------------------------------------------ range gate

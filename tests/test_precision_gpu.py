@@ -132,3 +132,24 @@ def test_phase_rereference_on_device(c1):
     assert np.array_equal(former(Sr), want)
     with pytest.raises(ValueError, match='one finite value per pulse'):
         former.rereference(coef[:-1])
+
+
+def test_phase_rereference_exact_former(c1):
+    """ExactFormer.rereference(coef): the same re-reference on the exact former's window pass equals the history
+    multiplied on the host; None removes it"""
+    col, S, grid = c1
+    rng = np.random.default_rng(10)
+    coef = 2 * rng.uniform(-3.0, 3.0, col.Np) / sim.C
+    f = col.fmin + col.df * np.arange(col.K)
+    Sr = (S * np.exp(-2j * np.pi * f[None, :] * coef[:, None])).astype(np.complex64)
+    former = ExactFormer(col.ant, col.fmin, col.df, col.K, **grid)
+    want = former(Sr)
+    former.rereference(coef)
+    got = former(S)
+    e = oracle.plain_db(got, want)
+    print(f'exact former, re-reference on the device: {e:.1f} dB from the host-side re-reference')
+    assert e < -110
+    former.rereference(None)
+    assert np.array_equal(former(Sr), want)
+    with pytest.raises(ValueError, match='one finite value per pulse'):
+        former.rereference(coef[:-1])

@@ -460,8 +460,9 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
     const float2* yb = y + (size_t)b * P * Ko + j;
     const h2* ybh = yh + (size_t)b * P * Ko + j;
     float2* zb = z + (size_t)b * Po * Ko + j;
-    // a deferred range rotation: sample (p, j) of row b times exp(2 pi i (c0 + (rD j - rpl - rkc) sl)) of row p
-    const double rj = (double)(rD * j - rpl) - rkc;
+    // a deferred range rotation: sample (p, j) of row b times exp(2 pi i (c0 + (rD j - rpl - rkc) sl)) of row p; the
+    // offset rD j - rpl - rkc is exact in float32 (an integer or a half below 2^23)
+    const float rj = (float)(rD * j - rpl) - (float)rkc;
     const float* rc0b = rc0 ? rc0 + (size_t)b * P : nullptr;
     const float* rslb = rsl ? rsl + (size_t)b * P : nullptr;
     float2 acc[NW];
@@ -483,9 +484,11 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
                         v = yb[(size_t)p * Ko];
                     }
                     if (rc0b) {
-                        // the phase in cycles wrapped in float64, then the hardware sine and cosine (revolutions)
-                        const double c = fma(rj, (double)rslb[p], (double)rc0b[p]);
-                        const float w = (float)(c - rint(c));
+                        // the phase in cycles: the product's rounding error recovered by an fma and the cycles wrapped
+                        // before the hardware sine and cosine (which take revolutions)
+                        const float b = rslb[p], pp = b * rj, pe = fmaf(b, rj, -pp);
+                        float w = (pp - rintf(pp)) + pe + rc0b[p];
+                        w -= rintf(w);
                         v = cmul(v, make_float2(__builtin_amdgcn_cosf(w), __builtin_amdgcn_sinf(w)));
                     }
                 }

@@ -438,7 +438,7 @@ __global__ __launch_bounds__(256) void k_fir_p(const float2* __restrict__ y, int
 // m - NW + 1 is complete, is stored, and the window shifts. The taps are the same for every lane (scalar loads, zero
 // beyond L). Consecutive threads take consecutive columns, so every row load is coalesced.
 // grid (ceil(B Ko / 256), ceil(Po / IC)).
-template <int NW, bool BATCH>
+template <int NW, bool BATCH, int RB>
 __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__ y, int B, int P, int Ko,
                                                        const float* __restrict__ taps, int L, int pl, int D, int Po, int IC,
                                                        float2* __restrict__ z, const float* __restrict__ hs,
@@ -473,30 +473,43 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
     for (int m = i0; m < i1 + NW - 1; ++m) {
         const int pbase = D * m - pl;
         if (!BATCH) {
-            for (int ph = 0; ph < D; ++ph) {
-                const int p = pbase + ph;
-                float2 v = make_float2(0.f, 0.f);
-                if (p >= 0 && p < P) {
-                    if (hs) {
-                        const h2 w = ybh[(size_t)p * Ko];
-                        v = make_float2((float)w.x * hinv, (float)w.y * hinv);
-                    } else {
-                        v = yb[(size_t)p * Ko];
-                    }
-                    if (rc0b) {
-                        // the phase in cycles: the product's rounding error recovered by an fma and the cycles wrapped
-                        // before the hardware sine and cosine (which take revolutions)
-                        const float b = rslb[p], pp = b * rj, pe = fmaf(b, rj, -pp);
-                        float w = (pp - rintf(pp)) + pe + rc0b[p];
-                        w -= rintf(w);
-                        v = cmul(v, make_float2(__builtin_amdgcn_cosf(w), __builtin_amdgcn_sinf(w)));
+            // the period's rows RB at a time: their loads are all issued before the first is used
+            for (int ph0 = 0; ph0 < D; ph0 += RB) {
+                float2 v[RB];
+                float rb[RB], ra[RB];
+                #pragma unroll
+                for (int u = 0; u < RB; ++u) {
+                    const int p = pbase + ph0 + u;
+                    v[u] = make_float2(0.f, 0.f);
+                    rb[u] = ra[u] = 0.f;
+                    if (ph0 + u < D && p >= 0 && p < P) {
+                        if (hs) {
+                            const h2 w = ybh[(size_t)p * Ko];
+                            v[u] = make_float2((float)w.x * hinv, (float)w.y * hinv);
+                        } else {
+                            v[u] = yb[(size_t)p * Ko];
+                        }
+                        if (rc0b) { rb[u] = rslb[p]; ra[u] = rc0b[p]; }
                     }
                 }
                 #pragma unroll
-                for (int s = 0; s < NW; ++s) {
-                    const float h = tpz[D * (NW - 1 - s) + ph];
-                    acc[s].x = fmaf(h, v.x, acc[s].x);
-                    acc[s].y = fmaf(h, v.y, acc[s].y);
+                for (int u = 0; u < RB; ++u) {
+                    const int ph = ph0 + u;
+                    if (ph >= D) break;
+                    if (rc0b) {
+                        // the phase in cycles: the product's rounding error recovered by an fma and the cycles wrapped
+                        // before the hardware sine and cosine (which take revolutions); rows outside [0, P) are zero
+                        const float pp = rb[u] * rj, pe = fmaf(rb[u], rj, -pp);
+                        float w = (pp - rintf(pp)) + pe + ra[u];
+                        w -= rintf(w);
+                        v[u] = cmul(v[u], make_float2(__builtin_amdgcn_cosf(w), __builtin_amdgcn_sinf(w)));
+                    }
+                    #pragma unroll
+                    for (int s = 0; s < NW; ++s) {
+                        const float h = tpz[D * (NW - 1 - s) + ph];
+                        acc[s].x = fmaf(h, v[u].x, acc[s].x);
+                        acc[s].y = fmaf(h, v[u].y, acc[s].y);
+                    }
                 }
             }
         } else {
@@ -1069,12 +1082,16 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
         const float* rsl = rot ? rot->sl : nullptr;
         const int rD = rot ? rot->D : 0, rpl = rot ? rot->pl : 0;
         const double rkc = rot ? rot->kc : 0.0;
+        const char* rbe = getenv("FSH_FIRP_ROWS");                      // 1: one row's load at a time (sweeps)
+        const bool rows4 = !(rbe && strcmp(rbe, "1") == 0);
         const size_t lds = batch ? 0 : (size_t)4 * D * NW;
 #define FSH_FP(NN)                                                                                                       \
     do {                                                                                                                 \
-        if (batch) hipLaunchKernelGGL((k_fir_p_stream<NN, true>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs, \
-                                      rc0, rsl, rD, rpl, rkc);                                                           \
-        else hipLaunchKernelGGL((k_fir_p_stream<NN, false>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs,   \
+        if (batch) hipLaunchKernelGGL((k_fir_p_stream<NN, true, 1>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, \
+                                      hs, rc0, rsl, rD, rpl, rkc);                                                       \
+        else if (rows4) hipLaunchKernelGGL((k_fir_p_stream<NN, false, 4>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po,  \
+                                           IC, z, hs, rc0, rsl, rD, rpl, rkc);                                           \
+        else hipLaunchKernelGGL((k_fir_p_stream<NN, false, 1>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs, \
                                 rc0, rsl, rD, rpl, rkc);                                                                 \
     } while (0)
         switch (NW) {

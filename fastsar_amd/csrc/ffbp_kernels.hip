@@ -1082,15 +1082,17 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
         const float* rsl = rot ? rot->sl : nullptr;
         const int rD = rot ? rot->D : 0, rpl = rot ? rot->pl : 0;
         const double rkc = rot ? rot->kc : 0.0;
-        const char* rbe = getenv("FSH_FIRP_ROWS");                      // 1: one row's load at a time (sweeps)
-        const bool rows4 = !(rbe && strcmp(rbe, "1") == 0);
+        const char* rbe = getenv("FSH_FIRP_ROWS");                      // rows whose loads go out together: 1, 4, 8 (sweeps)
+        const int rows = rbe ? atoi(rbe) : 4;
         const size_t lds = batch ? 0 : (size_t)4 * D * NW;
 #define FSH_FP(NN)                                                                                                       \
     do {                                                                                                                 \
         if (batch) hipLaunchKernelGGL((k_fir_p_stream<NN, true, 1>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, \
                                       hs, rc0, rsl, rD, rpl, rkc);                                                       \
-        else if (rows4) hipLaunchKernelGGL((k_fir_p_stream<NN, false, 4>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po,  \
-                                           IC, z, hs, rc0, rsl, rD, rpl, rkc);                                           \
+        else if (rows == 8) hipLaunchKernelGGL((k_fir_p_stream<NN, false, 8>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, \
+                                               Po, IC, z, hs, rc0, rsl, rD, rpl, rkc);                                   \
+        else if (rows == 4) hipLaunchKernelGGL((k_fir_p_stream<NN, false, 4>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, \
+                                               Po, IC, z, hs, rc0, rsl, rD, rpl, rkc);                                   \
         else hipLaunchKernelGGL((k_fir_p_stream<NN, false, 1>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs, \
                                 rc0, rsl, rD, rpl, rkc);                                                                 \
     } while (0)

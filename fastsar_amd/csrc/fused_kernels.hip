@@ -85,17 +85,40 @@ __global__ __launch_bounds__(256) void k_level_fused(const float2* __restrict__ 
         if (++pph == Dp) { end_period(); pph = 0; }
     }
 
+    // the first XPF samples per thread of each chunk are loaded a chunk ahead into registers (their latency hidden
+    // behind the chunk before), the rest when the chunk is staged
+    constexpr int XPF = 4;
+    float2 xpf[XPF];
+    auto prefetch = [&](int p0) {
+        const float2* src = xn + (size_t)p0 * K;
+        const int cnt = min(R, P - p0) * K;
+        #pragma unroll
+        for (int k = 0; k < XPF; ++k) {
+            const int i = tid + 256 * k;
+            xpf[k] = i < cnt ? src[i] : make_float2(0.f, 0.f);
+        }
+    };
+    prefetch(0);
     for (int p0 = 0; p0 < P; p0 += R) {
         const int rn = min(R, P - p0);
         {
             const float2* src = xn + (size_t)p0 * K;
             const int cnt = rn * K;
-            for (int i = tid; i < cnt; i += 256) {
+            #pragma unroll
+            for (int k = 0; k < XPF; ++k) {
+                const int i = tid + 256 * k;
+                if (i < cnt) {
+                    const int r = i / K;
+                    xs[r * KS + (i - r * K)] = xpf[k];
+                }
+            }
+            for (int i = tid + 256 * XPF; i < cnt; i += 256) {
                 const int r = i / K;
                 xs[r * KS + (i - r * K)] = src[i];
             }
         }
         __syncthreads();
+        if (p0 + R < P) prefetch(p0 + R);
 
         // range filter: thread (c, r)
         if (tid < cn * R) {

@@ -84,15 +84,34 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
         tab[i] = e;
     }
     const float2* xrow = x + ((size_t)n * P + p) * K;
-    for (int i = tid; i < NPH * WM; i += 256) {
-        const int ph = i / WM, m = i - ph * WM;
-        const int k = ks + D * m + ph;
-        const float2 v = (ph < D && k >= 0 && k < K) ? xrow[k] : make_float2(0.f, 0.f);
-        __bf16 rh, rl, ih, il;
-        split(v.x, rh, rl);
-        split(v.y, ih, il);
-        XH[ph * PS + m] = pack2(rh, ih);
-        XL[ph * PS + m] = pack2(rl, il);
+    // the window words i = tid + 256 k (word m of plane ph), XB loads issued before the first is split and stored
+    {
+        constexpr int XB = 8;
+        const int total = NPH * WM;
+        int ph = tid / WM, m = tid - ph * WM;
+        for (int i0 = tid; i0 < total; i0 += 256 * XB) {
+            float2 v[XB];
+            int dst[XB];
+            #pragma unroll
+            for (int k = 0; k < XB; ++k) {
+                const int i = i0 + 256 * k, kk = ks + D * m + ph;
+                v[k] = make_float2(0.f, 0.f);
+                if (i < total && ph < D && kk >= 0 && kk < K) v[k] = xrow[kk];
+                dst[k] = i < total ? ph * PS + m : -1;
+                m += 256;
+                while (m >= WM) { m -= WM; ++ph; }
+            }
+            #pragma unroll
+            for (int k = 0; k < XB; ++k) {
+                if (dst[k] >= 0) {
+                    __bf16 rh, rl, ih, il;
+                    split(v[k].x, rh, rl);
+                    split(v[k].y, ih, il);
+                    XH[dst[k]] = pack2(rh, ih);
+                    XL[dst[k]] = pack2(rl, il);
+                }
+            }
+        }
     }
     __syncthreads();
 

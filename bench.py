@@ -285,7 +285,8 @@ def main():
     ap.add_argument('--taps', default=None, help='learned decimation filters (an .npz of fastsar_amd.learned) for the image')
     ap.add_argument('--no-learned', action='store_true', help='skip the learned form beside the plain one')
     ap.add_argument('--train-taps', type=int, default=0, help='train filters of this many taps per phase on 2 scenes of '
-                    'the workload first (seeds 11, 12), saved to gpurun_out/learned_T<n>.npz, and form with them')
+                    'the workload first (seeds 11, 12), saved to --taps-out, and form with them')
+    ap.add_argument('--taps-out', default=None, help='where --train-taps saves its filters (default learned_T<n>.npz)')
     ap.add_argument('--precision', default='float32', choices=['float32', 'float16'],
                     help='float16: the two-pass levels store their intermediate rows in float16')
     ap.add_argument('--history', default='complex64', choices=['complex64', 'float16', 'bfloat16'],
@@ -293,7 +294,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU: --gpus {args.gpus} is not supported (the image-formation path needs no '
-              'collective; see DESIGN.md for how it would shard)', file=sys.stderr)
+              'collective: a sharded former would give each GPU a contiguous range of first-level children and gather '
+              'the image blocks)', file=sys.stderr)
         sys.exit(2)
 
     if args.batch:
@@ -335,8 +337,10 @@ def main():
         t0 = time.perf_counter()
         taps = learned.train_scenes(p0, c0, scenes(), args.train_taps, window=_window(col.Np, col.K), iters=6000, log=log)
         log(f'trained {args.train_taps} taps per phase in {time.perf_counter() - t0:.1f} s: {taps.history}')
-        os.makedirs(os.path.join(ROOT, 'gpurun_out'), exist_ok=True)
-        taps.save(os.path.join(ROOT, 'gpurun_out', f'learned_T{args.train_taps}.npz'))
+        out = args.taps_out or f'learned_T{args.train_taps}.npz'
+        if os.path.dirname(out):
+            os.makedirs(os.path.dirname(out), exist_ok=True)
+        taps.save(out)
     elif args.taps:
         from fastsar_amd import learned
         taps = learned.LearnedTaps.load(args.taps)

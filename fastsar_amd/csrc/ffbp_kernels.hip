@@ -888,7 +888,8 @@ int launch_prepare(const float2* S, int P, int K, const float* wp, const float* 
                    unsigned* xmax, const double* ramp)
 {
     const size_t n = (size_t)P * K;
-    const int blocks = (int)std::min<size_t>((n + 255) / 256, 1 << 16);
+    // with xmax, 4096 blocks (enough to keep HBM busy) so that the one atomic per wave stays a few thousand
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, xmax ? 4096 : 1 << 16);
     hipLaunchKernelGGL(k_prepare, dim3(blocks), dim3(256), 0, s, S, P, K, wp, wk, x, bad, xmax, (const double2*)ramp);
     FSH_TRY(hipGetLastError());
     return OK;
@@ -974,7 +975,7 @@ int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, c
                      int* bad, hipStream_t s, unsigned* xmax, const double* ramp)
 {
     const size_t n = (size_t)P * K;
-    const int blocks = (int)std::min<size_t>((n + 255) / 256, 1 << 16);
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, xmax ? 4096 : 1 << 16);
     if (dtype == 1)
         hipLaunchKernelGGL(k_prepare16<1>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax,
                            (const double2*)ramp);

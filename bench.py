@@ -451,26 +451,39 @@ def main():
             and args.history == 'complex64' and args.precision == 'float32' and os.path.exists(lpath)):
         from fastsar_amd import learned
         lt = learned.LearnedTaps.load(lpath)
-        lf = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T, levels=args.levels, atten=args.atten, taps=lt)
         limg = _hip.DeviceArray((grid['nx'], grid['ny']), np.complex64)
-        for _ in range(args.warmup):
-            lf(hist, out=limg)
-        _hip.check(_hip.lib().fsh_synchronize())
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            lf(hist, out=limg)
-        _hip.check(_hip.lib().fsh_synchronize())
-        ldt = (time.perf_counter() - t0) / args.steps
-        lh = limg.to_host()
+
+        def timed(precision):
+            # warm calls, then the timed ones; -> (former, seconds per image, error against the checker in dB)
+            f = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T, levels=args.levels, atten=args.atten,
+                            taps=lt, precision=precision)
+            for _ in range(args.warmup):
+                f(hist, out=limg)
+            _hip.check(_hip.lib().fsh_synchronize())
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                f(hist, out=limg)
+            _hip.check(_hip.lib().fsh_synchronize())
+            sec = (time.perf_counter() - t1) / args.steps
+            lh = limg.to_host()
+            lgot = np.concatenate([lh[i0:i0 + side, j0:j0 + side].ravel() for i0, j0 in tiles])
+            return f, sec, rel_db(lgot, ref_vals)
+
+        lf, ldt, lerr = timed('float32')
         lplan = lf._plan
-        lgot = np.concatenate([lh[i0:i0 + side, j0:j0 + side].ravel() for i0, j0 in tiles])
+        del lf
         learned_form = dict(ms_per_step=round(ldt * 1e3, 2), value=round(pairs / ldt / 1e9, 2),
-                            error_db=round(rel_db(lgot, ref_vals), 2), speedup=round(dt / ldt, 3),
+                            error_db=round(lerr, 2), speedup=round(dt / ldt, 3),
                             range_taps=[l['fir_k']['L'] if l['fir_k'] else 1 for l in lplan['levels']],
                             pulse_taps=[l['fir_p']['L'] if l['fir_p'] else 1 for l in lplan['levels']],
                             kaiser_kept=sorted(f'{a}[{b}]' for b, a in lt.keep))
+        # the same filters with the two-pass levels' rows stored in float16
+        hf, hdt, herr = timed('float16')
+        del hf
+        learned_form['float16'] = dict(ms_per_step=round(hdt * 1e3, 2), value=round(pairs / hdt / 1e9, 2),
+                                       error_db=round(herr, 2), speedup=round(dt / hdt, 3))
         log(f'learned form: {learned_form}')
-        del lf, limg
+        del limg
 
     cpu = None
     if not args.no_cpu:

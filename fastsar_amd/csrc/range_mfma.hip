@@ -44,6 +44,18 @@ __device__ __forceinline__ void split(float v, __bf16& hi, __bf16& lo)
     lo = (__bf16)(v - (float)hi);
 }
 
+// split of two values at once into packed words: hi = (bf16 a, bf16 b), lo = the bf16 of the remainders (the same
+// roundings as split)
+__device__ __forceinline__ void split_pair(float a, float b, unsigned int& hi, unsigned int& lo)
+{
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    const bf16x2_t h = {(__bf16)a, (__bf16)b};
+    hi = __builtin_bit_cast(unsigned int, h);
+    const float ha = __builtin_bit_cast(float, hi << 16), hb = __builtin_bit_cast(float, hi & 0xffff0000u);
+    const bf16x2_t l = {(__bf16)(a - ha), (__bf16)(b - hb)};
+    lo = __builtin_bit_cast(unsigned int, l);
+}
+
 // one pair of contraction indices (kappa, kappa + 1) = taps (r0, r0 + D) of phase ph: their values and the word
 // offset ph PS + u of their first sample in the window planes
 struct PairTab {
@@ -137,7 +149,7 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
         }
         // the N-tile's taps: B[kappa][column] for this lane's column (child c, Re or Im) and contraction indices
         // 8s + 4q .. 8s + 4q + 3; rows (Re x, Im x) of column Re y_c hold (Re g, -Im g), of column Im y_c (Im g, Re g)
-        bf16x8_t Gh[KS], Gl[KS];
+        uint4_t Gw[KS], Gv[KS];                                 // hi and lo operand words
         {
             float vdr, vdi;
             cis_cyc(b * (double)D, vdr, vdi);
@@ -158,13 +170,19 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
                     const float g0r = h0 * sr, g0i = h0 * si, g1r = h1 * s2r, g1i = h1 * s2i;
                     const float a0 = t ? g0i : g0r, b0 = t ? g0r : -g0i;
                     const float a1 = t ? g1i : g1r, b1 = t ? g1r : -g1i;
-                    __bf16 hh, ll;
-                    split(a0, hh, ll); Gh[s][4 * pr + 0] = hh; Gl[s][4 * pr + 0] = ll;
-                    split(b0, hh, ll); Gh[s][4 * pr + 1] = hh; Gl[s][4 * pr + 1] = ll;
-                    split(a1, hh, ll); Gh[s][4 * pr + 2] = hh; Gl[s][4 * pr + 2] = ll;
-                    split(b1, hh, ll); Gh[s][4 * pr + 3] = hh; Gl[s][4 * pr + 3] = ll;
+                    unsigned int h0w, l0w, h1w, l1w;
+                    split_pair(a0, b0, h0w, l0w);
+                    split_pair(a1, b1, h1w, l1w);
+                    if (pr == 0) { Gw[s].x = h0w; Gv[s].x = l0w; Gw[s].y = h1w; Gv[s].y = l1w; }
+                    else { Gw[s].z = h0w; Gv[s].z = l0w; Gw[s].w = h1w; Gv[s].w = l1w; }
                 }
             }
+        }
+        bf16x8_t Gh[KS], Gl[KS];
+        #pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            Gh[s] = __builtin_bit_cast(bf16x8_t, Gw[s]);
+            Gl[s] = __builtin_bit_cast(bf16x8_t, Gv[s]);
         }
         float sDr = 1.f, sDi = 0.f, s8r = 1.f, s8i = 0.f;
         if (ROT) {

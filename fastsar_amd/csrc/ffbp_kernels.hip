@@ -550,6 +550,100 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
     }
 }
 
+// The streaming pulse filter with two columns per thread, j and j + ceil(Ko / 2) of one row b: twice the row loads
+// in flight per wave and two independent accumulator windows, the rows of a period loaded four at a time. The same
+// sums, rotation and output as k_fir_p_stream.
+template <int NW>
+__global__ __launch_bounds__(256) void k_fir_p_stream2(const float2* __restrict__ y, int B, int P, int Ko,
+                                                        const float* __restrict__ taps, int L, int pl, int D, int Po, int IC,
+                                                        float2* __restrict__ z, const float* __restrict__ hs,
+                                                        const float* __restrict__ rc0, const float* __restrict__ rsl, int rD,
+                                                        int rpl, double rkc)
+{
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    constexpr int RB = 4;
+    const float hinv = hs ? hs[1] : 1.f;
+    extern __shared__ float tpz[];                   // [D * NW], zero beyond L
+    for (int r = threadIdx.x; r < D * NW; r += 256) tpz[r] = r < L ? taps[r] : 0.f;
+    __syncthreads();
+    const int H = (Ko + 1) / 2;
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long)B * H) return;
+    const int b = (int)(t / H), ja = (int)(t - (long)b * H), jb = ja + H;
+    const bool two = jb < Ko;
+    const int jb0 = two ? jb : ja;                   // the second column's loads stay inside the row when it is absent
+    const int i0 = blockIdx.y * IC, i1 = min(Po, i0 + IC);
+    const size_t base = (size_t)b * P * Ko;
+    const float2* ya = y + base + ja;
+    const float2* yb2 = y + base + jb0;
+    const h2* yha = reinterpret_cast<const h2*>(y) + base + ja;
+    const h2* yhb = reinterpret_cast<const h2*>(y) + base + jb0;
+    float2* za = z + (size_t)b * Po * Ko + ja;
+    float2* zb2 = z + (size_t)b * Po * Ko + jb0;
+    const float rja = (float)(rD * ja - rpl) - (float)rkc, rjb = (float)(rD * jb0 - rpl) - (float)rkc;
+    const float* rc0b = rc0 ? rc0 + (size_t)b * P : nullptr;
+    const float* rslb = rsl ? rsl + (size_t)b * P : nullptr;
+    float2 acca[NW], accb[NW];
+    #pragma unroll
+    for (int s = 0; s < NW; ++s) acca[s] = accb[s] = make_float2(0.f, 0.f);
+    auto rot = [&](float2 v, float bsl, float ac0, float rj) {
+        const float pp = bsl * rj, pe = fmaf(bsl, rj, -pp);
+        float w = (pp - rintf(pp)) + pe + ac0;
+        w -= rintf(w);
+        return cmul(v, make_float2(__builtin_amdgcn_cosf(w), __builtin_amdgcn_sinf(w)));
+    };
+    for (int m = i0; m < i1 + NW - 1; ++m) {
+        const int pbase = D * m - pl;
+        for (int ph0 = 0; ph0 < D; ph0 += RB) {
+            float2 va[RB], vb[RB];
+            float rb[RB], ra[RB];
+            #pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int p = pbase + ph0 + u;
+                va[u] = vb[u] = make_float2(0.f, 0.f);
+                rb[u] = ra[u] = 0.f;
+                if (ph0 + u < D && p >= 0 && p < P) {
+                    if (hs) {
+                        const h2 wa = yha[(size_t)p * Ko], wb = yhb[(size_t)p * Ko];
+                        va[u] = make_float2((float)wa.x * hinv, (float)wa.y * hinv);
+                        vb[u] = make_float2((float)wb.x * hinv, (float)wb.y * hinv);
+                    } else {
+                        va[u] = ya[(size_t)p * Ko];
+                        vb[u] = yb2[(size_t)p * Ko];
+                    }
+                    if (rc0b) { rb[u] = rslb[p]; ra[u] = rc0b[p]; }
+                }
+            }
+            #pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int ph = ph0 + u;
+                if (ph >= D) break;
+                float2 a = va[u], c = vb[u];
+                if (rc0b) {
+                    a = rot(a, rb[u], ra[u], rja);
+                    c = rot(c, rb[u], ra[u], rjb);
+                }
+                #pragma unroll
+                for (int s = 0; s < NW; ++s) {
+                    const float h = tpz[D * (NW - 1 - s) + ph];
+                    acca[s].x = fmaf(h, a.x, acca[s].x);
+                    acca[s].y = fmaf(h, a.y, acca[s].y);
+                    accb[s].x = fmaf(h, c.x, accb[s].x);
+                    accb[s].y = fmaf(h, c.y, accb[s].y);
+                }
+            }
+        }
+        const int done = m - NW + 1;
+        if (done >= i0 && done < i1) {
+            za[(size_t)done * Ko] = acca[0];
+            if (two) zb2[(size_t)done * Ko] = accb[0];
+        }
+        #pragma unroll
+        for (int s = 0; s < NW - 1; ++s) { acca[s] = acca[s + 1]; accb[s] = accb[s + 1]; }
+        acca[NW - 1] = accb[NW - 1] = make_float2(0.f, 0.f);
+    }
+}
+
 // the same sums read from global memory, for pulse filters too long for the LDS rows
 __global__ __launch_bounds__(256) void k_fir_p_global(const float2* __restrict__ y, int P, int Ko, const float* __restrict__ taps,
                                                        int L, int pl, int D, int Po, float2* __restrict__ z)
@@ -1085,6 +1179,25 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
         const double rkc = rot ? rot->kc : 0.0;
         const char* rbe = getenv("FSH_FIRP_ROWS");                      // rows whose loads go out together: 1, 4, 8 (sweeps)
         const int rows = rbe ? atoi(rbe) : 4;
+        // columns per thread: 2 where the launch has columns to spare (from 2^17: the 12288 x 8192 image's level 1,
+        // 44.6 -> 43.5 ms), else 1 (a 512 x 512 image's first pulse filter ran 0.14 ms on one, 0.17 on two)
+        const char* cle = getenv("FSH_FIRP_COLS");
+        const int cols = cle ? atoi(cle) : ((long)B * Ko >= (1L << 17) ? 2 : 1);
+        if (!batch && cols == 2) {
+            const int H = (Ko + 1) / 2;
+            dim3 grid2((unsigned)(((long)B * H + 255) / 256), (Po + IC - 1) / IC);
+#define FSH_FP2(NN) hipLaunchKernelGGL((k_fir_p_stream2<NN>), grid2, dim3(256), (size_t)4 * D * NW, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs, \
+                                       rc0, rsl, rD, rpl, rkc)
+            switch (NW) {
+                case 1: FSH_FP2(1); break;   case 2: FSH_FP2(2); break;   case 3: FSH_FP2(3); break;   case 4: FSH_FP2(4); break;
+                case 5: FSH_FP2(5); break;   case 6: FSH_FP2(6); break;   case 7: FSH_FP2(7); break;   case 8: FSH_FP2(8); break;
+                case 9: FSH_FP2(9); break;   case 10: FSH_FP2(10); break; case 11: FSH_FP2(11); break; case 12: FSH_FP2(12); break;
+                case 13: FSH_FP2(13); break; case 14: FSH_FP2(14); break; case 15: FSH_FP2(15); break; default: FSH_FP2(16); break;
+            }
+#undef FSH_FP2
+            FSH_TRY(hipGetLastError());
+            return OK;
+        }
         const size_t lds = batch ? 0 : (size_t)4 * D * NW;
 #define FSH_FP(NN)                                                                                                       \
     do {                                                                                                                 \

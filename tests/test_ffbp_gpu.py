@@ -211,23 +211,24 @@ def scene():
     return col, sim.simulate_brute(col, pos, amp).astype(np.complex64)
 
 
-@pytest.mark.parametrize('kernel', ['valu', 'mfma', 'fused', 'auto', 'lmfma', 'rot-in-range', 'rows1', 'rows8', 'cols1'])
+@pytest.mark.parametrize('kernel', ['valu', 'mfma', 'fused', 'auto', 'lmfma', 'rot-in-range', 'rows1', 'rows8', 'cols1',
+                                    'cols2'])
 @pytest.mark.parametrize('nlev,T', [(2, 16), (3, 16), (3, 8), (2, 32), (4, 4)])
 def test_image_vs_oracle(scene, nlev, T, kernel, monkeypatch):
     """whole images against the float64 oracle's image: float32 VALU levels, bfloat16 x 3 MFMA levels, MFMA level 0
     with the later levels fused on VALU where they fit, the default choice, and the later levels fused on the matrix
     cores where they fit; on the two-pass MFMA levels also with the range rotation kept in the range filter instead
-    of the pulse filter, and with the pulse filter on one column per thread, its rows loaded one, four or eight at a
-    time"""
+    of the pulse filter, and with the pulse filter on one column per thread (its rows loaded one, four or eight at a
+    time) or two"""
     if kernel == 'lmfma':
         monkeypatch.setenv('FSH_LEVEL_KERNEL', 'mfma')
-    elif kernel in ('rot-in-range', 'rows1', 'rows8', 'cols1'):
+    elif kernel in ('rot-in-range', 'rows1', 'rows8', 'cols1', 'cols2'):
         monkeypatch.setenv('FSH_ROT_KERNEL', 'mfma')
         monkeypatch.setenv('FSH_LEVEL_KERNEL', 'split')
         if kernel == 'rot-in-range':
             monkeypatch.setenv('FSH_DEFER_ROT', '0')
-        elif kernel == 'cols1':
-            monkeypatch.setenv('FSH_FIRP_COLS', '1')
+        elif kernel in ('cols1', 'cols2'):
+            monkeypatch.setenv('FSH_FIRP_COLS', kernel[4:])
         else:
             monkeypatch.setenv('FSH_FIRP_COLS', '1')
             monkeypatch.setenv('FSH_FIRP_ROWS', kernel[4:])

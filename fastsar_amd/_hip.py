@@ -57,6 +57,10 @@ def lib():
         'fsh_memset': ([vp, c_int, c_size_t], c_int),
         'fsh_synchronize': ([], c_int),
         'fsh_copy': ([vp, vp, c_size_t, vp], c_int),
+        'fsh_device_bytes': ([ctypes.POINTER(c_size_t)], c_int),
+        'fsh_mem_info': ([ctypes.POINTER(c_size_t), ctypes.POINTER(c_size_t)], c_int),
+        'fsh_ffbp_level_forms': ([vp, P_i32, c_int], c_int),
+        'fsh_ffbp_generation': ([vp, ctypes.POINTER(ctypes.c_ulonglong)], c_int),
         'fsh_rot_fir_k': ([vp, c_int, c_int, c_int, vp, vp, c_int, vp, c_int, c_int, c_int, c_int, c_double, vp, vp], c_int),
         'fsh_fir_p': ([vp, c_int, c_int, c_int, vp, c_int, c_int, c_int, c_int, vp, vp], c_int),
         'fsh_level_fused': ([vp, c_int, c_int, c_int, vp, vp, c_int, vp, c_int, c_int, c_int, c_int, vp, c_int, c_int, c_int,
@@ -126,6 +130,39 @@ def available():
         return True
     except Exception:
         return False
+
+
+def device_bytes():
+    """Device bytes the library holds now (every allocation it made and has not freed)."""
+    n = c_size_t()
+    check(lib().fsh_device_bytes(ctypes.byref(n)), 'device_bytes')
+    return n.value
+
+
+def mem_info():
+    """(free, total) device bytes."""
+    fr, to = c_size_t(), c_size_t()
+    check(lib().fsh_mem_info(ctypes.byref(fr), ctypes.byref(to)), 'mem_info')
+    return fr.value, to.value
+
+
+def output(out, shape):
+    """(buffer, on_device) for an image of `shape`: a new host array when out is None, else out itself, which must be
+    a complex64 DeviceArray or a writable C-contiguous complex64 numpy array of exactly that shape (ValueError
+    otherwise: the native side writes shape[0] * shape[1] * 8 bytes into it)."""
+    shape = tuple(int(n) for n in shape)
+    if out is None:
+        return np.empty(shape, np.complex64), False
+    if isinstance(out, DeviceArray):
+        if out.shape != shape or out.dtype != np.complex64:
+            raise ValueError(f'out must be complex64 {shape}, got a DeviceArray {out.dtype} {out.shape}')
+        return out, True
+    if isinstance(out, np.ndarray):
+        if out.shape != shape or out.dtype != np.complex64 or not out.flags.c_contiguous or not out.flags.writeable:
+            raise ValueError(f'out must be a writable C-contiguous complex64 array of shape {shape}, got {out.dtype} '
+                             f'{out.shape}')
+        return out, False
+    raise ValueError(f'out must be a complex64 DeviceArray or numpy array of shape {shape}, got {type(out).__name__}')
 
 
 def ptr(a):

@@ -240,6 +240,7 @@ class ImageFormer:
         coll = _plan.collection_arrays(plan, self.ant, ref)
         wspan = dict(P=self.P) if padded is None else dict(P=int(padded[1]), first=int(padded[0]))
         wf = None if aperture_weight is None else final_weights(plan, aperture_weight, grad=True, **wspan)
+        self._weighted = wf is not None
         if window:
             self.wp, self.wk = _window(self.P, self.K)
         win = (self.wp, self.wk) if window else None
@@ -302,17 +303,25 @@ class ImageFormer:
         return self._form(S, out=out)
 
     def memory(self):
-        """dict(backend, needed, available, full_speed, parts) in bytes. (fastsar/api.py:643-649)"""
+        """What full speed needs on the device and what is available to this former: dict(backend, needed, available,
+        full_speed, parts, group) in bytes (fastsar/api.py:643-649). needed follows the former's own buffers
+        (fastsar_amd.memory.hip_full_speed: a host history, its windowed copy, the image, the group buffers at the
+        default group, the plan constants; and the range gate's work buffers); available is the free device memory
+        plus what the former already holds; group is the first-level group the available memory allows."""
         from . import _hip
-        info = self._form.info()
-        dev = _hip.device_info()
-        parts = dict(history=8 * self.P * self._Kf, formation=info['device_bytes'])
-        if self.range_gate is not None:
+        forms = self._form.level_forms()
+        gated = self.range_gate is not None
+        kw = dict(host_input=not gated, weights=self._weighted, window=bool(self.window) and not gated)
+        need, parts = _mem.hip_full_speed(self._plan, forms, **kw)
+        if gated:
             g = self.range_gate
             chunk = max(1, min(self.P, (1 << 30) // (8 * (g['Kp'] + g['N'] + self.K))))
-            parts['range_gate'] = 8 * chunk * (g['Kp'] + g['N'] + self.K)
-        need = float(sum(parts.values()))
-        return dict(backend=self.backend, needed=need, available=float(dev['mem']), full_speed=need <= dev['mem'], parts=parts)
+            parts['range_gate'] = float(8 * chunk * (g['Kp'] + g['N'] + self.K) + 8 * self.P * g['kept'])
+            need += parts['range_gate']
+        free, _ = _hip.mem_info()
+        avail = float(free + self._form.info()['device_bytes'])
+        return dict(backend=self.backend, needed=float(need), available=avail, full_speed=need <= avail, parts=parts,
+                    group=_mem.hip_group(self._plan, forms, avail - parts.get('range_gate', 0.0), **kw))
 
 
 def form_image(S, ant, fmin, df, nx, ny, spx, spy, e1=(1.0, 0.0, 0.0), e2=(0.0, 1.0, 0.0), *, algorithm='ffbp',

@@ -100,8 +100,15 @@ class Batch:
             if f._form.info()['group'] < 1:
                 f(S)                                               # the group buffers are made by a first call
         self.formers, self.histories = list(formers), list(histories)
-        self.images = images if images is not None else [_hip.DeviceArray((f._form.nx, f._form.ny), np.complex64)
-                                                          for f in formers]
+        if images is not None:
+            if len(images) != n:
+                raise ValueError('a batch needs one image per former')
+            for f, im in zip(formers, images):
+                if not isinstance(im, _hip.DeviceArray):
+                    raise ValueError('batch images must be DeviceArrays (the graph writes them on the device)')
+                _hip.output(im, (f._form.nx, f._form.ny))
+        self.images = list(images) if images is not None else [_hip.DeviceArray((f._form.nx, f._form.ny), np.complex64)
+                                                               for f in formers]
         vp = ctypes.c_void_p
         hs = (vp * n)(*[f._form.h for f in formers])
         ss = (vp * n)(*[S.ptr for S in histories])

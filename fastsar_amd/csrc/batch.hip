@@ -23,6 +23,7 @@ __global__ void k_stamp(unsigned long long* ts, int i)
 
 struct Batch {
     std::vector<void*> formers;
+    std::vector<unsigned long long> gens;  // each former's buffer generation when the graph was captured
     hipStream_t origin = nullptr;
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> events;
@@ -106,6 +107,7 @@ int fsh_batch_create(void* const* formers, const void* const* S, void* const* im
     if (ce != hipSuccess) return fail(hip_status(ce, "hipStreamEndCapture"));
     const hipError_t ie = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
     if (ie != hipSuccess) return fail(hip_status(ie, "hipGraphInstantiate"));
+    for (int i = 0; i < n; ++i) b->gens.push_back(former_generation(formers[i]));
     *batch = b;
     return OK;
 }
@@ -114,6 +116,13 @@ int fsh_batch_run(void* batch, float* ms, float* done_ms, int* bad)
 {
     if (!batch) { set_error("batch_run: null batch"); return E_INVALID; }
     Batch* b = static_cast<Batch*>(batch);
+    // the graph holds each former's group buffers as they were at capture: refuse to replay onto buffers made since
+    for (int i = 0; i < b->n; ++i)
+        if (former_generation(b->formers[i]) != b->gens[i]) {
+            set_error("batch_run: former " + std::to_string(i) + " has made its group buffers again since the batch was "
+                      "captured (another group size); create the batch again");
+            return E_INVALID;
+        }
     const auto t0 = std::chrono::steady_clock::now();
     FSH_TRY(hipGraphLaunch(b->exec, b->origin));
     FSH_TRY(hipStreamSynchronize(b->origin));

@@ -22,7 +22,8 @@ namespace fsh {
 
 // ------------------------------------------------------------------------------------------------ prepare
 
-// x[p, k] = S[p, k] * wp[p] * wk[k] (wp == nullptr: copy); bad: the first pulse holding a NaN or inf
+// x[p, k] = S[p, k] * wp[p] * wk[k] (wp == nullptr: copy); bad: the first pulse holding a NaN or inf. The rows may be
+// a block of a longer history starting at pulse p0 (wp, ramp and the pointers offset by the caller; bad absolute)
 // with xmax: the largest component magnitude of x as well (float bits compared as unsigned, one atomic per wave;
 // a NaN or inf, which the scan reports, is left out)
 __device__ __forceinline__ void note_max(float m, unsigned* xmax)
@@ -34,14 +35,14 @@ __device__ __forceinline__ void note_max(float m, unsigned* xmax)
 
 __global__ __launch_bounds__(256) void k_prepare(const float2* __restrict__ S, int P, int K, const float* __restrict__ wp,
                                                   const float* __restrict__ wk, float2* __restrict__ x, int* __restrict__ bad,
-                                                  unsigned* __restrict__ xmax, const double2* __restrict__ ramp)
+                                                  unsigned* __restrict__ xmax, const double2* __restrict__ ramp, int p0)
 {
     const size_t n = (size_t)P * K;
     float m = 0.f;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         float2 v = S[i];
         const int p = (int)(i / K);
-        if (!(isfinite(v.x) && isfinite(v.y))) atomicMin(bad, p);
+        if (!(isfinite(v.x) && isfinite(v.y))) atomicMin(bad, p0 + p);
         v = apply_ramp(v, ramp, p, (int)(i - (size_t)p * K));
         if (wp) {
             const float g = wp[p] * wk[i - (size_t)p * K];
@@ -979,12 +980,12 @@ static int lds_limit()
 }
 
 int launch_prepare(const float2* S, int P, int K, const float* wp, const float* wk, float2* x, int* bad, hipStream_t s,
-                   unsigned* xmax, const double* ramp)
+                   unsigned* xmax, const double* ramp, int p0)
 {
     const size_t n = (size_t)P * K;
     // with xmax, 4096 blocks (enough to keep HBM busy) so that the one atomic per wave stays a few thousand
     const int blocks = (int)std::min<size_t>((n + 255) / 256, xmax ? 4096 : 1 << 16);
-    hipLaunchKernelGGL(k_prepare, dim3(blocks), dim3(256), 0, s, S, P, K, wp, wk, x, bad, xmax, (const double2*)ramp);
+    hipLaunchKernelGGL(k_prepare, dim3(blocks), dim3(256), 0, s, S, P, K, wp, wk, x, bad, xmax, (const double2*)ramp, p0);
     FSH_TRY(hipGetLastError());
     return OK;
 }
@@ -1022,14 +1023,14 @@ template <int DT>
 __global__ __launch_bounds__(256) void k_prepare16(const void* __restrict__ S, int P, int K, const float* __restrict__ inv,
                                                     const float* __restrict__ wp, const float* __restrict__ wk,
                                                     float2* __restrict__ x, int* __restrict__ bad, unsigned* __restrict__ xmax,
-                                                    const double2* __restrict__ ramp)
+                                                    const double2* __restrict__ ramp, int p0)
 {
     const size_t n = (size_t)P * K;
     float m = 0.f;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int p = (int)(i / K);
         float2 v = decode_sample<DT>(S, i, inv, p);
-        if (!(isfinite(v.x) && isfinite(v.y))) atomicMin(bad, p);
+        if (!(isfinite(v.x) && isfinite(v.y))) atomicMin(bad, p0 + p);
         v = apply_ramp(v, ramp, p, (int)(i - (size_t)p * K));
         if (wp) {
             const float g = wp[p] * wk[i - (size_t)p * K];
@@ -1066,22 +1067,22 @@ int launch_half_scales(const unsigned* xmax, const float* gain, int n, float* hs
 }
 
 int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, const float* wp, const float* wk, float2* x,
-                     int* bad, hipStream_t s, unsigned* xmax, const double* ramp)
+                     int* bad, hipStream_t s, unsigned* xmax, const double* ramp, int p0)
 {
     const size_t n = (size_t)P * K;
     const int blocks = (int)std::min<size_t>((n + 255) / 256, xmax ? 4096 : 1 << 16);
     if (dtype == 1)
         hipLaunchKernelGGL(k_prepare16<1>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax,
-                           (const double2*)ramp);
+                           (const double2*)ramp, p0);
     else if (dtype == 2)
         hipLaunchKernelGGL(k_prepare16<2>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax,
-                           (const double2*)ramp);
+                           (const double2*)ramp, p0);
     else if (dtype == 3)
         hipLaunchKernelGGL(k_prepare16<3>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax,
-                           (const double2*)ramp);
+                           (const double2*)ramp, p0);
     else if (dtype == 4)
         hipLaunchKernelGGL(k_prepare16<4>, dim3(blocks), dim3(256), 0, s, S, P, K, inv, wp, wk, x, bad, xmax,
-                           (const double2*)ramp);
+                           (const double2*)ramp, p0);
     else {
         set_error("prepare: unknown history type");
         return E_INVALID;

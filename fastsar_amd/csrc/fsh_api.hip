@@ -6,6 +6,7 @@
 // writes its tiles straight into the image. Intermediates live in two ping-pong buffers sized for the group.
 #include "fsh_common.h"
 #include "fsh_kernels.h"
+#include "fsh_host.h"
 #include "../../include/fsh.h"
 
 #include <algorithm>
@@ -15,6 +16,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 namespace fsh {
@@ -31,16 +34,30 @@ int hip_status(hipError_t e, const char* what)
 
 static std::atomic<long> g_alloc_count{0};
 
+// every live device allocation of the library and its size: fsh_device_bytes reports the total, which the memory
+// models of the formers are checked against
+static std::mutex g_live_mu;
+static std::unordered_map<void*, size_t> g_live;
+static size_t g_live_bytes = 0;
+
+size_t live_device_bytes()
+{
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    return g_live_bytes;
+}
+
 int dev_alloc(void** p, size_t bytes)
 {
     *p = nullptr;
     if (bytes == 0) bytes = 256;
     // test hooks: FSH_FAIL_ALLOC=n fails the n-th allocation of the process, FSH_FAIL_ALLOC_ABOVE=b every allocation
-    // of more than b bytes (a smaller device)
+    // of more than b bytes (a smaller device), FSH_FAIL_ALLOC_SIZE=b every allocation of exactly b bytes
     const char* fail = getenv("FSH_FAIL_ALLOC");
     const char* above = getenv("FSH_FAIL_ALLOC_ABOVE");
+    const char* size = getenv("FSH_FAIL_ALLOC_SIZE");     // every allocation of exactly this many bytes
     const long n = ++g_alloc_count;
-    if ((fail && atol(fail) > 0 && n == atol(fail)) || (above && atoll(above) > 0 && bytes > (size_t)atoll(above))) {
+    if ((fail && atol(fail) > 0 && n == atol(fail)) || (above && atoll(above) > 0 && bytes > (size_t)atoll(above)) ||
+        (size && atoll(size) > 0 && bytes == (size_t)atoll(size))) {
         set_error("device allocation failed (FSH_FAIL_ALLOC)");
         return E_NOMEM;
     }
@@ -53,12 +70,24 @@ int dev_alloc(void** p, size_t bytes)
         set_error(buf);
         return E_NOMEM;
     }
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    g_live[*p] = bytes;
+    g_live_bytes += bytes;
     return OK;
 }
 
 void dev_free(void* p)
 {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(g_live_mu);
+        auto it = g_live.find(p);
+        if (it != g_live.end()) {
+            g_live_bytes -= it->second;
+            g_live.erase(it);
+        }
+    }
+    (void)hipFree(p);
 }
 
 __global__ void k_fill_int(int* p, int n, int v)
@@ -83,12 +112,13 @@ int launch_fill_int(int* p, int n, int value, hipStream_t s)
 // ------------------------------------------------------------------------------------------------ former
 
 template <class T>
-static int upload(T** dst, const T* src, size_t count)
+static int upload(T** dst, const T* src, size_t count, size_t* tally = nullptr)
 {
     *dst = nullptr;
     if (!src || count == 0) return OK;
     int st = dev_alloc((void**)dst, count * sizeof(T));
     if (st) return st;
+    if (tally) *tally += count * sizeof(T);
     FSH_TRY(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
     return OK;
 }
@@ -143,6 +173,10 @@ struct Former {
     float2* image = nullptr;
     // per group
     int ng_req = 0, ng = 0;
+    int auto_cap = 0;              // the largest automatic group that fitted after a halving (0: none yet)
+    unsigned long long gen = 0;    // changes whenever the group buffers are made again (a captured graph holds them)
+    Staging stage;                 // pinned staging ring and copy stream of host transfers
+    size_t const_bytes = 0;        // device bytes of the plan constants uploaded at create
     float2* buf[2] = {nullptr, nullptr};
     size_t bufsz = 0;
     float* c0b = nullptr;
@@ -160,14 +194,26 @@ struct Former {
     std::vector<Rec> recs;         // one per launch of the last profiled call
     int cur_level = -1;
 
-    ~Former() { release_group(); release_all(); }
+    ~Former() { stage.release(); release_group(); release_all(); }
 
     void release_group()
     {
+        if (buf[0] || buf[1] || ux) ++gen;
         dev_free(buf[0]); dev_free(buf[1]); dev_free(c0b); dev_free(slb); dev_free(ux); dev_free(uy); dev_free(tgeo);
         buf[0] = buf[1] = nullptr; c0b = slb = nullptr; ux = uy = tgeo = nullptr;
         bufsz = phsz = geosz = 0;
         ng = 0;
+    }
+
+    // device bytes the former holds now: constants, history buffers, image, group buffers, precision and ramp tables
+    size_t held() const
+    {
+        size_t n = const_bytes + (x ? (size_t)p.P * p.K * sizeof(float2) : 0) + (hist ? (size_t)p.P * p.K * sizeof(float2) : 0) +
+                   (size_t)p.nx * p.ny * sizeof(float2) + sizeof(int) + (pinv ? (size_t)p.P * sizeof(float) : 0) +
+                   (ramp ? (size_t)p.P * 2 * sizeof(double) : 0);
+        if (xmax) n += sizeof(unsigned) + 3 * lv.size() * sizeof(float);
+        n += 2 * bufsz * sizeof(float2) + 2 * phsz * sizeof(float) + geosz * (2 * p.Pf + 3) * sizeof(double);
+        return n;
     }
 
     void release_all()
@@ -225,6 +271,7 @@ struct Former {
     int alloc_group(int g)
     {
         release_group();
+        ++gen;
         size_t mb, mp, tiles;
         group_sizes(g, mb, mp, tiles);
         int st;
@@ -546,7 +593,7 @@ int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, voi
     }
     if (hipEventCreate(&f->ev[0]) != hipSuccess || hipEventCreate(&f->ev[1]) != hipSuccess) return fail(E_RUNTIME);
     const float unit = 1.0f;
-    if ((st = upload(&f->one, &unit, 1))) return fail(st);
+    if ((st = upload(&f->one, &unit, 1, &f->const_bytes))) return fail(st);
     long long children = 1;
     for (int l = 0; l < p.nlev; ++l) {
         const fsh_level_desc& h = levels[l];
@@ -561,16 +608,16 @@ int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, voi
         L.h = h;
         L.h.taps_k = L.h.taps_p = L.h.c0 = L.h.sl = nullptr;
         L.h.u = L.h.r0 = L.h.d = L.h.ref = nullptr;
-        if ((st = upload(&L.taps_k, h.taps_k, h.Dk > 1 ? (size_t)h.Lk : 0))) return fail(st);
+        if ((st = upload(&L.taps_k, h.taps_k, h.Dk > 1 ? (size_t)h.Lk : 0, &f->const_bytes))) return fail(st);
         if (h.Dk > 1) L.taps_k_host.assign(h.taps_k, h.taps_k + h.Lk);
         if (h.Dp > 1) L.taps_p_host.assign(h.taps_p, h.taps_p + h.Lp);
-        if ((st = upload(&L.taps_p, h.taps_p, h.Dp > 1 ? (size_t)h.Lp : 0))) return fail(st);
+        if ((st = upload(&L.taps_p, h.taps_p, h.Dp > 1 ? (size_t)h.Lp : 0, &f->const_bytes))) return fail(st);
         if (h.c0) {
             const size_t n = (size_t)h.nparents * h.C * h.P;
-            if ((st = upload(&L.c0, h.c0, n)) || (st = upload(&L.sl, h.sl, n))) return fail(st);
+            if ((st = upload(&L.c0, h.c0, n, &f->const_bytes)) || (st = upload(&L.sl, h.sl, n, &f->const_bytes))) return fail(st);
         } else {
-            if ((st = upload(&L.u, h.u, (size_t)h.P * 3)) || (st = upload(&L.r0, h.r0, (size_t)h.P)) ||
-                (st = upload(&L.d, h.d, (size_t)h.C * 3)) || (st = upload(&L.ref, h.ref, (size_t)h.nparents * 3)))
+            if ((st = upload(&L.u, h.u, (size_t)h.P * 3, &f->const_bytes)) || (st = upload(&L.r0, h.r0, (size_t)h.P, &f->const_bytes)) ||
+                (st = upload(&L.d, h.d, (size_t)h.C * 3, &f->const_bytes)) || (st = upload(&L.ref, h.ref, (size_t)h.nparents * 3, &f->const_bytes)))
                 return fail(st);
         }
         {
@@ -578,6 +625,7 @@ int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, voi
             // short children whose parent one block holds, else on the matrix cores, else the two-pass form
             const int Lk = h.Dk > 1 ? h.Lk : 1, Lp = h.Dp > 1 ? h.Lp : 1;
             if (l > 0 && level_fused_preferred(h.C, h.P, h.K, Lk, h.Dk, h.Ko, Lp, h.Dp)) {
+                f->const_bytes += ((size_t)h.Dk * fused_nw(Lk, h.Dk) + (size_t)h.Dp * fused_nw(Lp, h.Dp)) * sizeof(float);
                 if ((st = dev_alloc((void**)&L.hk_t, (size_t)h.Dk * fused_nw(Lk, h.Dk) * sizeof(float))) ||
                     (st = dev_alloc((void**)&L.hp_t, (size_t)h.Dp * fused_nw(Lp, h.Dp) * sizeof(float))) ||
                     (st = launch_polyphase(h.Dk > 1 ? L.taps_k : nullptr, Lk, h.Dk, L.hk_t, f->stream)) ||
@@ -586,6 +634,7 @@ int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, voi
                 L.fused = true;
             } else if (l > 0 && level_mfma_preferred(h.C, Lk, h.Dk, h.Ko, Lp, h.Dp, h.Po)) {
                 const int nw = level_mfma_nw(Lp, h.Dp);
+                f->const_bytes += (size_t)h.Dp * nw * sizeof(float);
                 if ((st = dev_alloc((void**)&L.hp_t, (size_t)h.Dp * nw * sizeof(float))) ||
                     (st = launch_polyphase_nw(L.taps_p, Lp, h.Dp, nw, L.hp_t, f->stream)))
                     return fail(st);
@@ -599,13 +648,13 @@ int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, voi
         set_error("ffbp_create: the levels' children do not multiply to the tile count");
         return fail(E_INVALID);
     }
-    if ((st = upload(&f->wp, p.wp, p.wp ? (size_t)p.P : 0)) || (st = upload(&f->wk, p.wk, p.wk ? (size_t)p.K : 0)) ||
-        (st = upload(&f->fu, p.fu, (size_t)p.Pf * 3)) || (st = upload(&f->fr0, p.fr0, (size_t)p.Pf)) ||
-        (st = upload(&f->fcen, p.fcen, (size_t)p.ntiles * 3)) || (st = upload(&f->tile_xy, p.tile_xy, (size_t)p.ntiles * 2)))
+    if ((st = upload(&f->wp, p.wp, p.wp ? (size_t)p.P : 0, &f->const_bytes)) || (st = upload(&f->wk, p.wk, p.wk ? (size_t)p.K : 0, &f->const_bytes)) ||
+        (st = upload(&f->fu, p.fu, (size_t)p.Pf * 3, &f->const_bytes)) || (st = upload(&f->fr0, p.fr0, (size_t)p.Pf, &f->const_bytes)) ||
+        (st = upload(&f->fcen, p.fcen, (size_t)p.ntiles * 3, &f->const_bytes)) || (st = upload(&f->tile_xy, p.tile_xy, (size_t)p.ntiles * 2, &f->const_bytes)))
         return fail(st);
     if (p.w0) {
         const size_t n = (size_t)p.ntiles * p.Pf;
-        if ((st = upload(&f->w0, p.w0, n)) || (st = upload(&f->gx, p.gx, n)) || (st = upload(&f->gyr, p.gyr, n))) return fail(st);
+        if ((st = upload(&f->w0, p.w0, n, &f->const_bytes)) || (st = upload(&f->gx, p.gx, n, &f->const_bytes)) || (st = upload(&f->gyr, p.gyr, n, &f->const_bytes))) return fail(st);
     }
     f->p.wp = f->p.wk = nullptr;
     f->p.fu = f->p.fr0 = f->p.fcen = nullptr;
@@ -630,12 +679,7 @@ int fsh_ffbp_info(void* handle, int* ng_used, size_t* device_bytes)
     if (!handle) { set_error("ffbp_info: null handle"); return E_INVALID; }
     Former* f = static_cast<Former*>(handle);
     if (ng_used) *ng_used = f->ng;
-    if (device_bytes) {
-        size_t mb, mp, tiles;
-        f->group_sizes(std::max(f->ng, 1), mb, mp, tiles);
-        *device_bytes = (size_t)f->p.P * f->p.K * 8 * (f->hist ? 2 : 1) + (size_t)f->p.nx * f->p.ny * 8 + 2 * mb * 8 + 2 * mp * 4 +
-                        tiles * f->p.Pf * 16;
-    }
+    if (device_bytes) *device_bytes = f->held();
     return OK;
 }
 
@@ -667,6 +711,41 @@ int fsh_ffbp_stage_ms(void* handle, float* ms, int n)
     return OK;
 }
 
+// image rows [0, r) that the first-level children [0, upto) complete: child c = ix sy + iy covers rows
+// [ix cx - ox, (ix + 1) cx - ox) of the image (cx = Nx / sx), and children are formed in order
+static int rows_done(const Former* f, int upto)
+{
+    const fsh_level_desc& h = f->lv[0].h;
+    if (upto >= h.C) return f->p.nx;
+    const long long cx = f->p.Nx / h.sx;
+    const long long r = (long long)(upto / h.sy) * cx - f->p.ox;
+    return (int)std::min<long long>(f->p.nx, std::max<long long>(0, r));
+}
+
+// The group buffers: requested (set_group), else the default capped by the largest automatic group that fitted
+// before, halved while an allocation fails. The cap keeps a former that halved once from freeing and re-making its
+// buffers on every call (a batch graph captured on them would then replay freed memory).
+static int ensure_group(Former* f)
+{
+    const int G = f->lv[0].h.C;
+    const int dflt = f->default_group();
+    int want = f->ng_req > 0 ? std::min(f->ng_req, G) : (f->auto_cap > 0 ? std::min(dflt, f->auto_cap) : dflt);
+    if (f->ng == want) return OK;
+    while (true) {
+        const int st = f->alloc_group(want);
+        if (st == OK) break;
+        f->release_group();
+        if (st != E_NOMEM || want == 1 || f->ng_req > 0) return st;
+        want = std::max(1, want / 2);
+    }
+    if (f->ng_req == 0 && want < dflt) f->auto_cap = want;
+    return OK;
+}
+
+// One image. A host history goes up in blocks through the pinned staging ring, the window pass running on each block
+// as it lands; a host image comes down a band of rows at a time, each band as soon as the groups that form it are
+// done, beside the next group's kernels. The scan for NaN and inf is read at the end (no wait between the window pass
+// and the levels); a history that holds one returns FSH_ENONFINITE and the image is not defined.
 static int ffbp_form_impl(Former* f, const void* S, int dtype, const float* pulse_scale, int S_on_device, void* image,
                           int image_on_device, int* bad_pulse)
 {
@@ -676,32 +755,73 @@ static int ffbp_form_impl(Former* f, const void* S, int dtype, const float* puls
     f->recs.clear();
     f->cur_level = -1;
     const size_t ssz = dtype == FSH_C64 ? sizeof(float2) : dtype == FSH_CI8 ? 2 : 4;
-    const void* src = S;
+    if ((st = ensure_group(f))) return st;
+    if (!S_on_device && !f->hist && (st = dev_alloc((void**)&f->hist, (size_t)p.P * p.K * sizeof(float2)))) return st;
+    if ((!S_on_device || !image_on_device) && (st = f->stage.init())) return st;
+    auto fail = [&](int s) {
+        if (f->stage.ready()) f->stage.abandon();
+        return s;
+    };
     const float* inv = pulse_scale;
-    if (!S_on_device) {
-        if (!f->hist && (st = dev_alloc((void**)&f->hist, (size_t)p.P * p.K * sizeof(float2)))) return st;
-        FSH_TRY(hipMemcpyAsync(f->hist, S, (size_t)p.P * p.K * ssz, hipMemcpyHostToDevice, f->stream));
-        src = f->hist;
-        if (pulse_scale) {
-            if (!f->pinv && (st = dev_alloc((void**)&f->pinv, (size_t)p.P * sizeof(float)))) return st;
-            FSH_TRY(hipMemcpyAsync(f->pinv, pulse_scale, (size_t)p.P * sizeof(float), hipMemcpyHostToDevice, f->stream));
-            inv = f->pinv;
+    if (!S_on_device && pulse_scale) {
+        if (!f->pinv && (st = dev_alloc((void**)&f->pinv, (size_t)p.P * sizeof(float)))) return st;
+        FSH_TRY(hipMemcpyAsync(f->pinv, pulse_scale, (size_t)p.P * sizeof(float), hipMemcpyHostToDevice, f->stream));
+        inv = f->pinv;
+    }
+    // a device image is written in place by the final stage; a host image goes through the former's own
+    float2* own_image = f->image;
+    if (image_on_device) f->image = static_cast<float2*>(image);
+    struct Restore {
+        Former* f;
+        float2* img;
+        ~Restore() { f->image = img; }
+    } restore{f, own_image};
+    f->mark_begin();
+    if ((st = launch_fill_int(f->bad, 1, INT_MAX, f->stream))) return fail(st);
+    if (f->half_mid && (st = launch_fill_int((int*)f->xmax, 1, 0, f->stream))) return fail(st);
+    const double* rp = f->use_ramp ? f->ramp : nullptr;
+    auto prepare = [&](const void* src, int p0, int np) -> int {
+        const float* wpb = f->wp ? f->wp + p0 : nullptr;
+        const double* rpb = rp ? rp + 2 * (size_t)p0 : nullptr;
+        float2* xb = f->x + (size_t)p0 * p.K;
+        if (dtype == FSH_C64)
+            return launch_prepare((const float2*)src, np, p.K, wpb, f->wk, xb, f->bad, f->stream, f->xmax, rpb, p0);
+        return launch_prepare16(src, dtype, inv ? inv + p0 : nullptr, np, p.K, wpb, f->wk, xb, f->bad, f->stream, f->xmax, rpb,
+                                p0);
+    };
+    if (S_on_device) {
+        if ((st = prepare(S, 0, p.P))) return fail(st);
+    } else {
+        const size_t row = (size_t)p.K * ssz;
+        st = f->stage.upload(f->hist, S, (size_t)p.P * row, row, f->stream, [&](size_t off, size_t n) {
+            const int p0 = (int)(off / row);
+            return prepare(reinterpret_cast<const char*>(f->hist) + off, p0, (int)(n / row));
+        });
+        if (st) return fail(st);
+    }
+    if (f->half_mid && (st = launch_half_scales(f->xmax, f->hgain, (int)f->lv.size(), f->hsc, f->stream))) return fail(st);
+    f->mark_end(ST_PREP);
+    const int G = f->lv[0].h.C;
+    int rows = 0, gi = 0;
+    const size_t rowb = (size_t)p.ny * sizeof(float2);
+    for (int g0 = 0; g0 < G; g0 += f->ng, ++gi) {
+        const int gn = std::min(f->ng, G - g0);
+        if ((st = f->run_group(g0, gn))) return fail(st);
+        if (!image_on_device) {
+            const int r = rows_done(f, g0 + gn);
+            if (r > rows) {
+                if ((st = f->stage.mark(f->stream, gi)) ||
+                    (st = f->stage.download(static_cast<char*>(image) + (size_t)rows * rowb,
+                                            reinterpret_cast<const char*>(f->image) + (size_t)rows * rowb, (size_t)(r - rows) * rowb, gi)))
+                    return fail(st);
+                rows = r;
+            }
         }
     }
-    f->mark_begin();
-    if ((st = launch_fill_int(f->bad, 1, INT_MAX, f->stream))) return st;
-    if (f->half_mid && (st = launch_fill_int((int*)f->xmax, 1, 0, f->stream))) return st;
-    const double* rp = f->use_ramp ? f->ramp : nullptr;
-    if (dtype == FSH_C64) {
-        if ((st = launch_prepare((const float2*)src, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream, f->xmax, rp))) return st;
-    } else if ((st = launch_prepare16(src, dtype, inv, p.P, p.K, f->wp, f->wk, f->x, f->bad, f->stream, f->xmax, rp))) {
-        return st;
-    }
-    if (f->half_mid && (st = launch_half_scales(f->xmax, f->hgain, (int)f->lv.size(), f->hsc, f->stream))) return st;
-    f->mark_end(ST_PREP);
-    int badh = INT_MAX;
-    FSH_TRY(hipMemcpyAsync(&badh, f->bad, sizeof(int), hipMemcpyDeviceToHost, f->stream));
+    if (!image_on_device && (st = f->stage.drain())) return fail(st);
     FSH_TRY(hipStreamSynchronize(f->stream));
+    int badh = INT_MAX;
+    FSH_TRY(hipMemcpy(&badh, f->bad, sizeof(int), hipMemcpyDeviceToHost));
     if (badh != INT_MAX) {
         if (bad_pulse) *bad_pulse = badh;
         char buf[200];
@@ -709,28 +829,6 @@ static int ffbp_form_impl(Former* f, const void* S, int dtype, const float* puls
         set_error(buf);
         return E_NONFINITE;
     }
-    // the group: requested, else the default (default_group), halved while the buffers do not fit
-    const int G = f->lv[0].h.C;
-    int want = f->ng_req > 0 ? std::min(f->ng_req, G) : f->default_group();
-    if (f->ng != want) {
-        while (true) {
-            st = f->alloc_group(want);
-            if (st == OK) break;
-            f->release_group();
-            if (st != E_NOMEM || want == 1 || f->ng_req > 0) return st;
-            want = std::max(1, want / 2);
-        }
-    }
-    for (int g0 = 0; g0 < G; g0 += f->ng) {
-        const int gn = std::min(f->ng, G - g0);
-        if ((st = f->run_group(g0, gn))) return st;
-    }
-    if (image_on_device) {
-        FSH_TRY(hipMemcpyAsync(image, f->image, (size_t)p.nx * p.ny * sizeof(float2), hipMemcpyDeviceToDevice, f->stream));
-    } else {
-        FSH_TRY(hipMemcpyAsync(image, f->image, (size_t)p.nx * p.ny * sizeof(float2), hipMemcpyDeviceToHost, f->stream));
-    }
-    FSH_TRY(hipStreamSynchronize(f->stream));
     return OK;
 }
 
@@ -773,10 +871,18 @@ int fsh_ffbp_set_precision(void* handle, int half_intermediates)
     }
     const int n = (int)f->lv.size();
     if (!f->xmax) {
-        int st;
-        if ((st = dev_alloc((void**)&f->xmax, sizeof(unsigned))) || (st = dev_alloc((void**)&f->hsc, 2 * n * sizeof(float))) ||
-            (st = dev_alloc((void**)&f->hgain, n * sizeof(float))))
+        // all three tables or none: a failure leaves the former as it was (float32 rows)
+        unsigned* xm = nullptr;
+        float* sc = nullptr;
+        float* gn = nullptr;
+        auto undo = [&](int st) {
+            dev_free(xm); dev_free(sc); dev_free(gn);
             return st;
+        };
+        int st;
+        if ((st = dev_alloc((void**)&xm, sizeof(unsigned))) || (st = dev_alloc((void**)&sc, 2 * n * sizeof(float))) ||
+            (st = dev_alloc((void**)&gn, n * sizeof(float))))
+            return undo(st);
         // the bound of level l's range-filtered rows over the windowed history's largest component: the L1 norms of
         // every filter before them (the rotations keep magnitudes); two components, so times sqrt 2 for the modulus
         std::vector<float> g(n, 0.f);
@@ -789,7 +895,11 @@ int fsh_ffbp_set_precision(void* handle, int half_intermediates)
             g[l] = (float)(acc * nk);
             acc *= nk * np;
         }
-        FSH_TRY(hipMemcpy(f->hgain, g.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        const hipError_t e = hipMemcpy(gn, g.data(), n * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return undo(hip_status(e, "ffbp_set_precision"));
+        f->xmax = xm;
+        f->hsc = sc;
+        f->hgain = gn;
     }
     f->half_mid = true;
     return OK;
@@ -822,6 +932,37 @@ int fsh_ffbp_enqueue(void* handle, const void* S, void* image, void* stream)
     return st;
 }
 
+int fsh_mem_info(size_t* free_bytes, size_t* total_bytes)
+{
+    size_t fr = 0, to = 0;
+    FSH_TRY(hipMemGetInfo(&fr, &to));
+    if (free_bytes) *free_bytes = fr;
+    if (total_bytes) *total_bytes = to;
+    return OK;
+}
+
+int fsh_ffbp_generation(void* handle, unsigned long long* gen)
+{
+    if (!handle || !gen) { set_error("ffbp_generation: null argument"); return E_INVALID; }
+    *gen = static_cast<Former*>(handle)->gen;
+    return OK;
+}
+
+int fsh_ffbp_level_forms(void* handle, int* forms, int n)
+{
+    if (!handle || !forms) { set_error("ffbp_level_forms: null argument"); return E_INVALID; }
+    const Former* f = static_cast<Former*>(handle);
+    for (int l = 0; l < n && l < (int)f->lv.size(); ++l) forms[l] = f->lv[l].mfma ? 2 : f->lv[l].fused ? 1 : 0;
+    return OK;
+}
+
+int fsh_device_bytes(size_t* bytes)
+{
+    if (!bytes) { set_error("device_bytes: null argument"); return E_INVALID; }
+    *bytes = live_device_bytes();
+    return OK;
+}
+
 int fsh_ffbp_bad_pulse(void* handle, int* bad_pulse)
 {
     if (!handle || !bad_pulse) { set_error("ffbp_bad_pulse: null argument"); return E_INVALID; }
@@ -843,3 +984,9 @@ int fsh_ffbp_destroy(void* handle)
 }
 
 }  // extern "C"
+
+namespace fsh {
+
+unsigned long long former_generation(void* handle) { return static_cast<Former*>(handle)->gen; }
+
+}  // namespace fsh

@@ -6,13 +6,15 @@
 namespace fsh {
 
 // ramp: [P] (a, b) in cycles, the phase re-reference exp(2 pi i (a + b k)) applied to every sample before the window
+// p0: the rows are pulses p0, p0 + 1, ... of a longer history (bad holds absolute pulse indices; the caller offsets
+// S, wp, ramp, inv and x)
 int launch_prepare(const float2* S, int P, int K, const float* wp, const float* wk, float2* x, int* bad, hipStream_t s,
-                   unsigned* xmax = nullptr, const double* ramp = nullptr);
+                   unsigned* xmax = nullptr, const double* ramp = nullptr, int p0 = 0);
 int launch_half_scales(const unsigned* xmax, const float* gain, int n, float* hs, hipStream_t s);
 // the history types of fsh_ffbp_form_typed and fsh_exact_form_typed with their scale rule: OK or E_INVALID (message set)
 int check_history_type(int dtype, const void* pulse_scale);
 int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, const float* wp, const float* wk, float2* x,
-                     int* bad, hipStream_t s, unsigned* xmax = nullptr, const double* ramp = nullptr);
+                     int* bad, hipStream_t s, unsigned* xmax = nullptr, const double* ramp = nullptr, int p0 = 0);
 int launch_fill_int(int* p, int n, int value, hipStream_t s);
 // deferred != nullptr: the caller's pulse filter can apply rho (a launch_fir_p with the same c0, sl, D, pl, kc), and
 // *deferred tells whether the chosen kernel left it out of y
@@ -67,8 +69,12 @@ int launch_final_tiles(const float2* data, int B, int Pf, int Qf, int T, const d
 // tiles (a power of two, 1 for 65536 pixel blocks and more)
 int final_split(long tiles, int T);
 
-// device allocation through one helper, so that a test can make the n-th allocation fail (FSH_FAIL_ALLOC=n)
+// device allocation through one helper, so that a test can make the n-th allocation fail (FSH_FAIL_ALLOC=n); the live
+// allocations and their total are kept (live_device_bytes)
 int dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);
+size_t live_device_bytes();
+// a former's buffer generation: it changes whenever the former makes its group buffers again
+unsigned long long former_generation(void* handle);
 
 }  // namespace fsh

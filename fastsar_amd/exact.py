@@ -270,8 +270,7 @@ class ExactFormer:
         if isinstance(S, TypedHistory):
             if S.shape != (self.P, self.K):
                 raise ValueError(f'phase history must be {(self.P, self.K)}, got {S.shape}')
-            img_dev = isinstance(out, _hip.DeviceArray)
-            img = out if img_dev else np.empty((self.nx, self.ny), np.complex64)
+            img, img_dev = _hip.output(out, (self.nx, self.ny))
             bad = ctypes.c_int(-1)
             h = self._handle()
             data = S.data if S.on_device else np.ascontiguousarray(S.data)
@@ -287,9 +286,8 @@ class ExactFormer:
             S = np.ascontiguousarray(S, np.complex64)
         elif S.shape != (self.P, self.K) or S.dtype != np.complex64:
             raise ValueError(f'phase history must be complex64 {(self.P, self.K)}')
+        img, img_dev = _hip.output(out, (self.nx, self.ny))
         h = self._handle()
-        img_dev = isinstance(out, _hip.DeviceArray)
-        img = out if img_dev else np.empty((self.nx, self.ny), np.complex64)
         bad = ctypes.c_int(-1)
         _hip.check(_bind().fsh_exact_form(h, _hip.ptr(S), int(on_dev), _hip.ptr(img), int(img_dev), ctypes.byref(bad)),
                    'exact_form')

@@ -119,6 +119,19 @@ class HipFFBP:
         _hip.check(_hip.lib().fsh_ffbp_info(self.h, ctypes.byref(ng), ctypes.byref(nb)), 'ffbp_info')
         return dict(group=ng.value, device_bytes=nb.value)
 
+    def generation(self):
+        """A counter that changes whenever the former makes its group buffers again."""
+        g = ctypes.c_ulonglong()
+        _hip.check(_hip.lib().fsh_ffbp_generation(self.h, ctypes.byref(g)), 'ffbp_generation')
+        return g.value
+
+    def level_forms(self):
+        """How each level runs: 0 two passes, 1 one pass on VALU, 2 one pass on the matrix cores."""
+        n = len(self.plan['levels'])
+        v = (ctypes.c_int * n)()
+        _hip.check(_hip.lib().fsh_ffbp_level_forms(self.h, v, n), 'ffbp_level_forms')
+        return list(v[:])
+
     def set_profile(self, on):
         """Per-launch device timing (events on the former's stream, each launch waited for) of the next calls."""
         _hip.check(_hip.lib().fsh_ffbp_set_profile(self.h, int(bool(on))), 'ffbp_set_profile')
@@ -141,8 +154,7 @@ class HipFFBP:
         on the device; out as in __call__."""
         if h.shape != (self.P, self.K):
             raise ValueError(f'phase history must be {(self.P, self.K)}, got {h.shape}')
-        img_dev = isinstance(out, _hip.DeviceArray)
-        img = out if img_dev else np.empty((self.nx, self.ny), np.complex64)
+        img, img_dev = _hip.output(out, (self.nx, self.ny))
         bad = ctypes.c_int(-1)
         data = h.data if h.on_device else np.ascontiguousarray(h.data)
         inv = None if h.inv is None else (h.inv if h.on_device else np.ascontiguousarray(h.inv, np.float32))
@@ -151,8 +163,8 @@ class HipFFBP:
         return img
 
     def __call__(self, S, out=None):
-        """S [P, K] complex64 (host, C-contiguous) or a DeviceArray; out: None (a new host image) or a DeviceArray
-        [nx, ny] complex64 to write into. Returns the image."""
+        """S [P, K] complex64 (host, C-contiguous) or a DeviceArray; out: None (a new host image), or a complex64
+        [nx, ny] DeviceArray or numpy array to write into. Returns the image."""
         on_dev = isinstance(S, _hip.DeviceArray)
         if on_dev:
             if S.shape != (self.P, self.K) or S.dtype != np.complex64:
@@ -161,8 +173,7 @@ class HipFFBP:
             S = np.ascontiguousarray(S, np.complex64)
             if S.shape != (self.P, self.K):
                 raise ValueError(f'phase history must be {(self.P, self.K)}, got {S.shape}')
-        img_dev = isinstance(out, _hip.DeviceArray)
-        img = out if img_dev else np.empty((self.nx, self.ny), np.complex64)
+        img, img_dev = _hip.output(out, (self.nx, self.ny))
         bad = ctypes.c_int(-1)
         _hip.check(_hip.lib().fsh_ffbp_form(self.h, _hip.ptr(S), int(on_dev), _hip.ptr(img), int(img_dev), ctypes.byref(bad)),
                    'ffbp_form')

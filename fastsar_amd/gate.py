@@ -111,7 +111,7 @@ class RangeGate:
                 raise ValueError(f'phase history must be {(self.P, self.K)}, got {S.shape}')
         if out is None:
             out = _hip.DeviceArray((self.P, self.kept), np.complex64)
-        elif out.shape != (self.P, self.kept) or out.dtype != np.complex64:
+        elif not isinstance(out, _hip.DeviceArray) or out.shape != (self.P, self.kept) or out.dtype != np.complex64:
             raise ValueError(f'out must be a DeviceArray {(self.P, self.kept)} complex64')
         bad = ctypes.c_int(-1)
         _hip.check(L.fsh_gate_run(ctypes.c_void_p(self._h), _hip.ptr(S), int(on_dev), _hip.ptr(out), ctypes.byref(bad)),

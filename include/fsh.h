@@ -49,6 +49,11 @@ int fsh_synchronize(void);
 /* A plain device-to-device copy kernel (16-byte aligned, a multiple of 16 bytes): the reference point for the
  * HBM byte counters of the other kernels. No reference counterpart. */
 int fsh_copy(void* dst, const void* src, size_t bytes, void* stream);
+/* Device bytes the library holds now (every allocation it made and has not freed): the measure the formers' memory
+ * models are checked against. No reference counterpart (fastsar/memory.py sizes the reference's backends). */
+int fsh_device_bytes(size_t* bytes);
+/* Free and total device memory (hipMemGetInfo): what the formers' memory() reports as available. */
+int fsh_mem_info(size_t* free_bytes, size_t* total_bytes);
 
 /* ------------------------------------------------------------------------------------------ stages (device pointers)
  * stream: a hipStream_t, or NULL for the default stream. */
@@ -140,6 +145,11 @@ int fsh_ffbp_create(const fsh_plan_desc* plan, const fsh_level_desc* levels, voi
 /* S [P, K] complex (host or device), image [nx, ny] complex (host or device). bad_pulse: the first pulse with a
  * non-finite sample when FSH_ENONFINITE is returned. */
 int fsh_ffbp_form(void* handle, const void* S, int S_on_device, void* image, int image_on_device, int* bad_pulse);
+/* A host history goes up in blocks of pulses through a ring of pinned staging buffers (FSH_STAGE_MB, default 64 MB,
+ * three of them; FSH_COPY_THREADS host threads fill them), the window pass running on each block as it lands; a host
+ * image comes down in bands of rows as the groups that complete them finish, beside the next group's kernels (the
+ * reference's pinned double buffers and copy threads, fastsar/ffbp_cuda.py:692-745). A device image is written in
+ * place. The scan for NaN and inf is read at the end: with FSH_ENONFINITE the image is not defined. */
 /* First-level children per group (0: automatic, up to 8, halved while device memory lacks). */
 /* The same with a history in a given type, decoded to float32 on the device before the window:
  *   FSH_C64      complex64 (as fsh_ffbp_form), pulse_scale NULL;
@@ -167,8 +177,15 @@ int fsh_ffbp_set_precision(void* handle, int half_intermediates);
  * float64 (host) in cycles, sample k of pulse p times exp(2 pi i (a_p + b_p k)) before the window; NULL removes it. */
 int fsh_ffbp_set_ramp(void* handle, const double* ab);
 int fsh_ffbp_set_group(void* handle, int ng);
-/* The group used by the last call and the device bytes the former holds. */
+/* The group used by the last call and the device bytes the former holds (plan constants, history buffers, image, group
+ * buffers, precision and ramp tables). */
 int fsh_ffbp_info(void* handle, int* ng_used, size_t* device_bytes);
+/* How each level runs: 0 two passes (range filter, then pulse filter, through an intermediate), 1 one pass on VALU,
+ * 2 one pass on the matrix cores. The memory model needs it: a two-pass level holds its intermediate. */
+int fsh_ffbp_level_forms(void* handle, int* forms, int n);
+/* A counter that changes whenever the former makes its group buffers again (another group size): a graph captured on
+ * the former (fsh_batch_create) is valid while it is unchanged. No reference counterpart. */
+int fsh_ffbp_generation(void* handle, unsigned long long* gen);
 /* Per-stage device times (ms) of the last call: prepare, rot_fir_k, fir_p, device_phases, final_geom, final_tiles,
  * level_fused, level_mfma. */
 int fsh_ffbp_stage_ms(void* handle, float* ms, int n);

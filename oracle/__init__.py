@@ -169,6 +169,35 @@ def bp_points(S, ant, fmin, df, pts, ref=None, upsample=64, interp='linear', chu
     return (ore + 1j * oim).reshape(shape)
 
 
+def bp_points_weighted(S, ant, fmin, df, pts, weight, upsample=32, interp='cubic'):
+    """bp_points with a weight per point and pulse: sum_p weight(pts, [p])[0, n] prof_p(t_pn) exp(2 pi i kcyc dR_pn),
+    the image an aperture weight asks for (fastsar/api.py:203-244 approximates it per final subaperture and tile).
+    weight(points [n, 3], pulses [m] int) -> [m, n]. -> complex128 [...]."""
+    S = np.asarray(S)
+    ant = np.asarray(ant, np.float64)
+    P, K = S.shape
+    ref = np.linalg.norm(ant, axis=1)
+    pts = np.asarray(pts, np.float64)
+    shape = pts.shape[:-1]
+    flat = np.ascontiguousarray(pts.reshape(-1, 3))
+    nfft = 1 << int(np.ceil(np.log2(upsample * K)))
+    inv_dr = 2.0 * df * nfft / C
+    kcyc = 2.0 * (fmin + (K // 2) * df) / C
+    out = np.zeros(len(flat), np.complex128)
+    L = lib()
+    for p0 in range(0, P, 64):
+        p1 = min(P, p0 + 64)
+        rc = range_compress(S[p0:p1], nfft)
+        w = np.asarray(weight(flat, np.arange(p0, p1)), np.float64)
+        for i in range(p1 - p0):
+            ore, oim = np.zeros(len(flat)), np.zeros(len(flat))
+            L.orc_bp_points(np.ascontiguousarray(rc[i:i + 1].real), np.ascontiguousarray(rc[i:i + 1].imag), 1, nfft,
+                            np.ascontiguousarray(ant[p0 + i:p0 + i + 1]), np.ascontiguousarray(ref[p0 + i:p0 + i + 1]),
+                            flat, len(flat), inv_dr, kcyc, 3 if interp == 'cubic' else 1, ore, oim)
+            out += w[i] * (ore + 1j * oim)
+    return out.reshape(shape)
+
+
 def direct_sum(S, ant, fmin, df, pts, ref=None):
     """The interpolation-free image sum_p sum_k S[p, k] exp(+j 4 pi f_k dR / c) (complex128), for tiny cases."""
     S = np.asarray(S, np.complex128)

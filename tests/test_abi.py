@@ -95,3 +95,21 @@ def test_plan_descriptor_layout(tmp_path):
         assert got[(cname, 'size')] == ctypes.sizeof(cls), cname
         for f, _ in cls._fields_:
             assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
+
+
+def test_output_buffers_checked():
+    """an image buffer given as out= is refused unless it is complex64 of exactly the image's shape (the library
+    writes nx * ny * 8 bytes into it)"""
+    from fastsar_amd import _hip
+    ok = np.empty((4, 6), np.complex64)
+    assert _hip.output(ok, (4, 6)) == (ok, False)
+    img, dev = _hip.output(None, (4, 6))
+    assert img.shape == (4, 6) and img.dtype == np.complex64 and not dev
+    for bad in (np.empty((6, 4), np.complex64), np.empty((4, 6), np.complex128), np.empty((4, 12), np.complex64)[:, ::2],
+                np.empty(24, np.complex64), [[0j] * 6] * 4):
+        with pytest.raises(ValueError, match='out must be'):
+            _hip.output(bad, (4, 6))
+    ro = np.empty((4, 6), np.complex64)
+    ro.flags.writeable = False
+    with pytest.raises(ValueError, match='out must be'):
+        _hip.output(ro, (4, 6))

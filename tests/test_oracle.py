@@ -166,6 +166,36 @@ def test_bp_points_vs_numpy_reference():
     assert db(got, want) < -250
 
 
+def test_bp_points_weighted():
+    """the per-point, per-pulse weighted backprojection: unit weights give bp_points; weights that depend on the
+    pulse alone give bp_points of the history times those weights; the image is linear in the weight; a weight that
+    is one at a single point and zero elsewhere leaves only that point's value"""
+    col, tg, amp, S = _scene()
+    P = S.shape[0]
+    pts = np.concatenate([tg, np.array([[1.1, 2.2, 0.0], [-4.0, 3.0, 0.5], [0.3, -7.0, 0.0]])])
+    plain = oracle.bp_points(S, col.ant, col.fmin, col.df, pts, upsample=16, interp='cubic')
+    ones = oracle.bp_points_weighted(S, col.ant, col.fmin, col.df, pts, lambda q, p: np.ones((len(p), len(q))), upsample=16)
+    assert db(ones, plain) < -250
+    hw = 0.5 - 0.5 * np.cos(2 * np.pi * (np.arange(P) + 0.5) / P)
+    pw = oracle.bp_points_weighted(S, col.ant, col.fmin, col.df, pts, lambda q, p: np.repeat(hw[p][:, None], len(q), 1),
+                                   upsample=16)
+    assert db(pw, oracle.bp_points(S * hw[:, None], col.ant, col.fmin, col.df, pts, upsample=16, interp='cubic')) < -250
+
+    def wa(q, p):
+        return np.outer(np.cos(np.asarray(p) / 7.0), 1.0 + 0.01 * q[:, 0])
+
+    def wb(q, p):
+        return np.outer(np.asarray(p) / P, np.exp(-0.02 * q[:, 1]))
+    both = oracle.bp_points_weighted(S, col.ant, col.fmin, col.df, pts, lambda q, p: wa(q, p) + wb(q, p), upsample=16)
+    sep = (oracle.bp_points_weighted(S, col.ant, col.fmin, col.df, pts, wa, upsample=16) +
+           oracle.bp_points_weighted(S, col.ant, col.fmin, col.df, pts, wb, upsample=16))
+    assert db(both, sep) < -250
+    one = oracle.bp_points_weighted(S, col.ant, col.fmin, col.df, pts,
+                                    lambda q, p: np.repeat((np.arange(len(q)) == 2)[None, :].astype(float), len(p), 0),
+                                    upsample=16)
+    assert np.all(one[np.arange(len(pts)) != 2] == 0) and abs(one[2] / plain[2] - 1) < 1e-12
+
+
 def test_bp_oversampling_rate():
     """linear reads: error against a 128x reference falls about 12 dB per doubling (fastsar/bp.py:15-16)"""
     col, tg, amp, S = _scene(r0=4e3)

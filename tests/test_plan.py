@@ -165,3 +165,55 @@ def test_tile_digits_cover_grid():
         assert len({tuple(v) for v in xy}) == len(xy) == (Nx // T) * (Ny // T)
         assert xy[:, 0].min() == 0 and xy[:, 1].min() == 0 and xy[:, 0].max() == Nx - T and xy[:, 1].max() == Ny - T
         assert np.all(xy % T == 0)
+
+
+def _golden():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'plans.json')) as fh:
+        return json.load(fh)
+
+
+GOLDEN = _golden()
+
+
+@pytest.mark.parametrize('case', GOLDEN['plans'], ids=[c['name'] for c in GOLDEN['plans']])
+def test_plan_against_golden_table(case):
+    """the planner's decisions against the committed table (tests/golden/make_plans.py wrote it with the line-by-line
+    restatement of fastsar/ffbp2.py): every integer exactly, pass bands, frequencies and predicted errors to 1e-12,
+    the taps to 1e-12 of their largest, sampled host rotations to float32 rounding"""
+    from tests.golden.make_plans import cases
+    col, grid, kw = {c[0]: c[1:] for c in cases()}[case['name']]
+    p = pl.make_plan(col, grid['nx'], grid['ny'], grid['spx'], grid['spy'], e1=grid['e1'], e2=grid['e2'], **kw)
+    assert [list(s) for s in p['split']] == case['split']
+    assert (p['Nx'], p['Ny'], p['ox'], p['oy']) == (case['Nx'], case['Ny'], case['ox'], case['oy'])
+    for lv, want in zip(p['levels'], case['levels'], strict=True):
+        for k in ('sx', 'sy', 'C', 'Dk', 'Dp', 'K', 'P', 'Ko', 'Po', 'mk', 'mp'):
+            assert lv[k] == want[k], (k, lv[k], want[k])
+        assert len(lv['pidx']) == want['npidx'] and lv['pidx'][0] == want['pidx0']
+        for k in ('pass_k', 'pass_p', 'f0', 'df'):
+            assert lv[k] == pytest.approx(want[k], rel=1e-12, abs=1e-300), k
+        for ax in ('k', 'p'):
+            got, w = lv[f'fir_{ax}'], want[f'fir_{ax}']
+            assert (got is None) == (w is None)
+            if got is not None:
+                assert (got['L'], got['pl'], got['pr'], got['n_out']) == (w['L'], w['pl'], w['pr'], w['n_out'])
+                kw_ = np.asarray(w['kern'])
+                assert np.abs(got['kern'] - kw_).max() <= 1e-12 * np.abs(kw_).max()
+    f = p['final']
+    assert (f['P'], f['K'], f['cen'].shape[0]) == (case['final']['P'], case['final']['K'], case['final']['ntiles'])
+    for k in ('f0', 'df', 'fc'):
+        assert f[k] == pytest.approx(case['final'][k], rel=1e-12)
+    fmax = col.fmin + col.K * col.df
+    args = (col.ant, fmax, grid['nx'], grid['ny'], grid['spx'], grid['spy'], grid['e1'], grid['e2'])
+    assert pl.final_phase_error(*args, kw['T']) == pytest.approx(case['predicted_db'], abs=1e-9)
+    T, e = pl.choose_T(*args, -40.0)
+    assert T == case['auto_T'] and e == pytest.approx(case['auto_db'], abs=1e-9)
+    coll = pl.collection_arrays(p, col.ant)
+    for i, rot in enumerate(case['rotations']):
+        c0, sl = coll['levels'][i]['c0'], coll['levels'][i]['slope']
+        for t, a, b in zip(rot['idx'], rot['c0'], rot['slope']):
+            t = tuple(t)
+            assert abs((float(c0[t]) - a + 0.5) % 1.0 - 0.5) <= 2.0 ** -24 and float(sl[t]) == pytest.approx(b, rel=2.0 ** -23)
+    r0 = coll['final']['r0']
+    assert np.allclose(r0[::max(1, len(r0) // 8)], case['final_r0'], rtol=1e-15, atol=0)

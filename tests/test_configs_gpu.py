@@ -7,8 +7,10 @@ and the image of groups of 4 first-level children equal bit for bit to that of g
 
 48 uneven images (sides 256 to 4,096 px, 512 to 8,192 pulses x 1,024 to 4,096 samples, spotlight and 2-degree
 squint, seed 5): each image's final tile chosen by the predicted final-stage error (T 8, else 4, four levels; no
-timing), every replay of the batch graph equal bit for bit to the images of single calls, and every image within
--60 dB of the float64 checker on tiles about two of its targets."""
+timing) with 4 dB kept in hand, since the prediction can read low by that much (image 11, 534^2 from 1400 x 4088:
+-63.2 dB predicted at T 8, -59.2 dB measured, the same in float64 on the CPU and on the device), every replay of
+the batch graph equal bit for bit to the images of single calls, and every image within -60 dB of the float64 checker
+on tiles about two of its targets."""
 import numpy as np
 import pytest
 
@@ -100,7 +102,7 @@ def test_uneven_batch_48_images():
     for i, sc in enumerate(scenes):
         col, g = sc['col'], sc['grid']
         fmax = col.fmin + col.K * col.df
-        T, _ = pl.choose_T(col.ant, fmax, g['nx'], g['ny'], g['spx'], g['spy'], g['e1'], g['e2'], -60.0, choices=(8, 4))
+        T, _ = pl.choose_T(col.ant, fmax, g['nx'], g['ny'], g['spx'], g['spy'], g['e1'], g['e2'], -64.0, choices=(8, 4))
         hists.append(simulate_device(col, sc['pos'], sc['amp'], clutter_sigma=sc['sigma'], seed=100 + i))
         formers.append(ImageFormer(col.ant, col.fmin, col.df, col.K, **g, T=T, levels=4, atten=80.0))
     print(f'48 images, sides {min(s["grid"]["nx"] for s in scenes)}-{max(s["grid"]["nx"] for s in scenes)}', flush=True)

@@ -67,6 +67,8 @@ def lib():
                              c_int, c_double, vp, vp], c_int),
         'fsh_level_mfma': ([vp, c_int, c_int, c_int, vp, vp, c_int, vp, c_int, c_int, c_int, c_int, vp, c_int, c_int, c_int,
                             c_int, c_double, vp, vp], c_int),
+        'fsh_level_mfma_noalloc': ([vp, c_int, c_int, c_int, vp, vp, c_int, vp, c_int, c_int, c_int, c_int, vp, c_int, c_int,
+                                    c_int, c_int, c_double, vp, vp], c_int),
         'fsh_device_phases': ([vp, vp, vp, vp, c_int, c_int, c_int, c_double, c_double, vp, vp, vp], c_int),
         'fsh_final_geom': ([vp, vp, vp, c_int, c_int, P_f64, P_f64, vp, vp, vp, vp], c_int),
         'fsh_final_tiles': ([vp, c_int, c_int, c_int, c_int, vp, vp, vp, c_double, c_double, c_double, c_double, c_double,

@@ -432,6 +432,30 @@ int check_history_type(int dtype, const void* pulse_scale)
     return OK;
 }
 
+// The no-allocation variant's table: one per device for the life of the process. Its device address is looked up
+// once per device (hipGetSymbolAddress, with that device current) and only that address is handed to kernels; the
+// name itself is a host-side stand-in.
+static constexpr int NOALLOC_TABLE_FLOATS = 4096;
+static __device__ __align__(256) float scratch[NOALLOC_TABLE_FLOATS];
+static std::mutex g_scratch_mu;
+static std::vector<float*> g_scratch_ptr;            // by device ordinal
+
+static int scratch_table(float** out)
+{
+    int dev = 0;
+    FSH_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_status(e != hipSuccess ? e : hipErrorInvalidSymbol, "level_mfma_noalloc: scratch address");
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return OK;
+}
+
 }  // namespace fsh
 
 using namespace fsh;
@@ -543,6 +567,38 @@ int fsh_level_mfma(const float* x, int Np, int P, int K, const float* c0, const 
         if (e != hipSuccess) st = hip_status(e, "level_mfma");
     }
     dev_free(t);
+    return st;
+}
+
+int fsh_level_mfma_noalloc(const float* x, int Np, int P, int K, const float* c0, const float* sl, int Cn,
+                           const float* taps_k, int Lk, int plk, int Dk, int Ko, const float* taps_p, int Lp, int plp, int Dp,
+                           int Po, double kc, float* z, void* stream)
+{
+    if (!x || !c0 || !sl || !z || !taps_k || !taps_p) { set_error("level_mfma_noalloc: null argument"); return E_INVALID; }
+    int KS, NWp, NPH, PS, JT, njr, nseg, IS, nwp, tpw, nwv;
+    size_t lds;
+    if (Np < 1 || Lk < 1 || Lp < 1 || !level_mfma_shape(Np, Cn, Lk, Dk, Ko, Lp, Dp, Po, KS, NWp, NPH, PS, JT, njr, nseg, IS,
+                                                        lds, nwp, tpw, nwv)) {
+        set_error("level_mfma_noalloc: the level does not fit the fused matrix-core form (Dk, Dp >= 2, Ko >= 16, "
+                  "Dk ceil(Lk / Dk) <= 128, ceil(Lp / Dp) <= 16)");
+        return E_INVALID;
+    }
+    if ((long)Dp * nwp > NOALLOC_TABLE_FLOATS) {
+        set_error("level_mfma_noalloc: the pulse filter's polyphase table (" + std::to_string((long)Dp * nwp) +
+                  " floats) exceeds the fixed table of " + std::to_string(NOALLOC_TABLE_FLOATS));
+        return E_INVALID;
+    }
+    float* t = nullptr;
+    int st = scratch_table(&t);
+    if (st) return st;
+    // the whole table [Dp, nwp] is written by the polyphase kernel before the level reads it, on the same stream
+    const hipStream_t s = (hipStream_t)stream;
+    if (!(st = launch_polyphase_nw(taps_p, Lp, Dp, nwp, t, s)) &&
+        !(st = launch_level_mfma((const float2*)x, Np, P, K, c0, sl, Cn, taps_k, Lk, Dk, plk, Ko, t, Lp, Dp, plp, Po, kc,
+                                 (float2*)z, s))) {
+        const hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = hip_status(e, "level_mfma_noalloc");
+    }
     return st;
 }
 

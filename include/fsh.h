@@ -87,6 +87,16 @@ int fsh_level_mfma(const float* x, int Np, int P, int K, const float* c0, const 
                    int Lk, int plk, int Dk, int Ko, const float* taps_p, int Lp, int plp, int Dp, int Po, double kc, float* z,
                    void* stream);
 
+/* fsh_level_mfma without any allocation: the same kernels and results, the pulse filter's polyphase table built in a
+ * fixed device table of 4096 floats (16 KB per GPU, held until the process ends) instead of device memory allocated
+ * per call. Refuses (FSH_EINVAL) a level whose table Dp * level_mfma_nw(Lp, Dp) exceeds 4096 floats. Every call on
+ * one GPU shares that table: calls that overlap on one GPU (other host threads, other streams) race.
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). */
+int fsh_level_mfma_noalloc(const float* x, int Np, int P, int K, const float* c0, const float* sl, int Cn,
+                           const float* taps_k, int Lk, int plk, int Dk, int Ko, const float* taps_p, int Lp, int plp, int Dp,
+                           int Po, double kc, float* z, void* stream);
+
 /* Child rotations of a level computed on the device in float64: refs [Np, 3] parent centres, u [P, 3], r0 [P] the
  * level's track, d [Cn, 3] child offsets -> c0, sl [Np, Cn, P] float32. Replaces the device-phase kernel and the
  * host form (fastsar/ffbp_cuda.py:598-632, fastsar/ffbp_cpu.py:142-155). */

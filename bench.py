@@ -1,8 +1,8 @@
 #!/usr/bin/env python3
 """Benchmark of the hot path: factorized backprojection of a 12,288 x 8,192 spotlight image from 16,384 pulses x
 8,192 samples (complex64) on one MI355X, with the error against a float64 exact backprojection on 16 seeded
-256 x 256 tiles, the dominant kernel's roofline fraction, and the reference's CPU former timed on a sample of the
-same image on the host's cores.
+256 x 256 tiles, the dominant kernel's roofline fraction, exact backprojection of the same history (the whole grid and
+a 4,096 x 4,096 grid), and the reference's CPU former timed on the whole image on the host's cores.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--T 8]
 
@@ -82,13 +82,26 @@ def target_tiles(grid, targets, side):
     return tiles, np.concatenate(pts), side
 
 
-def cpu_baseline(col, grid, S_host, groups, T_cpu=16):
-    """The reference's CPU former (its own compiled kernels) on a sample of the first-level children: the window
-    pass over the whole history plus `groups` groups of 8 children, extrapolated to the whole image."""
+def cpu_model():
+    try:
+        with open('/proc/cpuinfo') as fh:
+            for line in fh:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpu_baseline(col, grid, S_host, images, tiles, side, ref_vals, T_cpu=16):
+    """The reference's CPU former (its own compiled kernels, fastsar/ffbp_cpu.cpp, driven as fastsar/ffbp_cpu.py:158-291
+    drives it) on the whole image, host to host (window pass, then every first-level child in groups of 8): after an
+    untimed warm call on one child, the median of `images` full images, and the error of the last one on the accuracy
+    tiles against the float64 checker."""
     import oracle
     from fastsar_amd import plan as pl
     from fastsar_amd.api import _window
-    threads = os.cpu_count() or 1
+    from fastsar_amd.metrics import rel_db
     os.environ.setdefault('OMP_PLACES', 'cores')
     os.environ.setdefault('OMP_PROC_BIND', 'close')
     L = oracle.ref_lib()
@@ -99,21 +112,65 @@ def cpu_baseline(col, grid, S_host, groups, T_cpu=16):
     G = p['levels'][0]['C']
     wp, wk = _window(col.Np, col.K)
     former(S_host[:, :], children=[G - 1], ng=1)                   # warm: one child, untimed
-    t0 = time.perf_counter()
-    Sw = former.window(S_host, wp, wk)
-    t_win = time.perf_counter() - t0
-    kids = list(range(min(G, 8 * groups)))
-    t0 = time.perf_counter()
-    former(Sw, children=kids, ng=8)
-    t_kids = time.perf_counter() - t0
-    t_full = t_win + t_kids * G / len(kids)
+    times, img = [], None
+    for _ in range(max(1, images)):
+        t0 = time.perf_counter()
+        img = former(former.window(S_host, wp, wk), ng=8)
+        times.append(time.perf_counter() - t0)
+        log(f'cpu baseline: one image in {times[-1]:.2f} s')
+    t = float(np.median(times))
+    got = np.concatenate([img[i0:i0 + side, j0:j0 + side].ravel() for i0, j0 in tiles])
+    err = rel_db(got, ref_vals)
     pairs = grid['nx'] * grid['ny'] * col.Np
-    return dict(value=round(pairs / t_full / 1e9, 3), unit='Gpixel·pulses/s', cores=threads, kind='reference',
+    return dict(value=round(pairs / t / 1e9, 3), unit='Gpixel·pulses/s', cores=threads, kind='reference',
                 sample=(f'reference CPU former (fastsar/ffbp_cpu.cpp built from its sources, x86-64-{L.isa}, OpenMP '
-                        f'{threads} threads), T={T_cpu}: window pass over the whole history ({t_win:.2f} s) plus '
-                        f'{len(kids)} of {G} first-level children ({t_kids:.2f} s), extrapolated to '
-                        f'{t_full:.2f} s per image'),
-                s_per_image=round(t_full, 3))
+                        f'{threads} threads on {cpu_model()}), T={T_cpu}, 3 levels: the whole image host to host, '
+                        f'median of {len(times)} after a warm call on one first-level child, no extrapolation'),
+                s_per_image=round(t, 3), times_s=[round(v, 3) for v in times], error_db=round(err, 2),
+                error_tiles=f'{len(tiles)} x {side}^2', cpu_model=cpu_model(), threads=threads)
+
+
+def exact_leg(col, grid, dS, tiles, side, ref_vals, steps):
+    """Exact backprojection (ExactFormer) of the same history: the whole grid with cubic reads (time of one warm call
+    and its error on the accuracy tiles against the float64 checker), and a 4,096 x 4,096 grid about the scene centre
+    with cubic and with linear reads (median of `steps` calls). Pixel-pulse pairs per second."""
+    from fastsar_amd import _hip
+    from fastsar_amd.exact import ExactFormer
+    from fastsar_amd.metrics import rel_db
+    out = {}
+    f = ExactFormer(col.ant, col.fmin, col.df, col.K, **grid)
+    img = _hip.DeviceArray((grid['nx'], grid['ny']), np.complex64)
+    f(dS, out=img)
+    _hip.check(_hip.lib().fsh_synchronize())
+    t0 = time.perf_counter()
+    f(dS, out=img)
+    _hip.check(_hip.lib().fsh_synchronize())
+    t = time.perf_counter() - t0
+    h = img.to_host()
+    err = rel_db(np.concatenate([h[i0:i0 + side, j0:j0 + side].ravel() for i0, j0 in tiles]), ref_vals)
+    pairs = grid['nx'] * grid['ny'] * col.Np
+    out['full'] = dict(interp='cubic', tile=list(f.tile), ms=round(t * 1e3, 1), gpairs_per_s=round(pairs / t / 1e9, 1),
+                       error_db=round(err, 2), error_tiles=f'{len(tiles)} x {side}^2')
+    log(f'exact backprojection {grid["nx"]}x{grid["ny"]} from {col.Np} x {col.K}, cubic: {out["full"]}')
+    del img, h, f
+    g4 = dict(grid, nx=4096, ny=4096)
+    img = _hip.DeviceArray((4096, 4096), np.complex64)
+    for interp in ('cubic', 'linear'):
+        f = ExactFormer(col.ant, col.fmin, col.df, col.K, **g4, interp=interp)
+        f(dS, out=img)
+        ts = []
+        for _ in range(max(1, steps)):
+            _hip.check(_hip.lib().fsh_synchronize())
+            t0 = time.perf_counter()
+            f(dS, out=img)
+            _hip.check(_hip.lib().fsh_synchronize())
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        pairs = 4096 * 4096 * col.Np
+        out[f'4096x4096_{interp}'] = dict(ms=round(t * 1e3, 2), gpairs_per_s=round(pairs / t / 1e9, 1))
+        log(f'exact backprojection 4096x4096 from {col.Np} x {col.K}, {interp}: {out[f"4096x4096_{interp}"]}')
+        del f
+    return out
 
 
 def measured_traffic(label, former, args):
@@ -269,8 +326,9 @@ def main():
     ap.add_argument('--levels', type=int, default=4, help='levels of the plan (the former API default is 3, more when 3 cannot split)')
     ap.add_argument('--tiles', type=int, default=16, help='accuracy tiles (256 x 256) against float64 backprojection')
     ap.add_argument('--upsample', type=int, default=64, help='oversampling of the float64 checker profiles')
-    ap.add_argument('--cpu-groups', type=int, default=2, help='groups of 8 first-level children timed on the CPU')
-    ap.add_argument('--host-steps', type=int, default=2, help='also time calls from a host history to a host image')
+    ap.add_argument('--cpu-images', type=int, default=1, help='whole images the reference CPU former forms (median)')
+    ap.add_argument('--host-steps', type=int, default=5, help='also time calls from a host history to a host image (median)')
+    ap.add_argument('--no-exact', action='store_true', help='skip the exact-backprojection leg')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-profile', action='store_true', help='timed steps only (for counter runs under rocprofv3)')
     ap.add_argument('--calibrate-copy', action='store_true', help='also run the plain copy kernel on 1 GiB three times')
@@ -358,25 +416,30 @@ def main():
         former(hist, out=img)
     _hip.check(_hip.lib().fsh_synchronize())
     log(f'warm: group of {former._form.info()["group"]} first-level children')
+    step_s = []
     t0 = time.perf_counter()
     for i in range(args.steps):
+        t1 = time.perf_counter()
         former(hist, out=img)
-    _hip.check(_hip.lib().fsh_synchronize())
+        _hip.check(_hip.lib().fsh_synchronize())
+        step_s.append(time.perf_counter() - t1)
     dt = (time.perf_counter() - t0) / args.steps
     pairs = grid['nx'] * grid['ny'] * col.Np
-    log(f'{dt * 1e3:.1f} ms per image, {pairs / dt / 1e9:.1f} Gpixel.pulses/s')
+    log(f'{dt * 1e3:.1f} ms per image (median {np.median(step_s) * 1e3:.1f}), {pairs / dt / 1e9:.1f} Gpixel.pulses/s')
 
     # host to host (pageable history in, image out): the PCIe-inclusive figure, reported beside the device rate
     h2h = None
     if args.host_steps > 0 and not args.no_profile:
         S_host = dS.to_host() if hist_host is None else hist_host
         former(S_host)
-        t0 = time.perf_counter()
+        hs = []
         for i in range(args.host_steps):
+            t0 = time.perf_counter()
             former(S_host)
-        h2h = (time.perf_counter() - t0) / args.host_steps
+            hs.append(time.perf_counter() - t0)
+        h2h = float(np.median(hs))
         del S_host
-        log(f'host to host: {h2h * 1e3:.1f} ms per image ({pairs / h2h / 1e9:.1f} Gpixel.pulses/s)')
+        log(f'host to host: median {h2h * 1e3:.1f} ms per image of {len(hs)} ({pairs / h2h / 1e9:.1f} Gpixel.pulses/s)')
 
     if args.calibrate_copy:
         # the plain copy kernel on 1 GiB, three times: the byte counters' reference point
@@ -485,13 +548,17 @@ def main():
         log(f'learned form: {learned_form}')
         del limg
 
+    exact = None
+    if not args.no_exact and not args.small and args.history == 'complex64':
+        exact = exact_leg(col, grid, dS, tiles, side, ref_vals, args.steps)
+
     cpu = None
     if not args.no_cpu:
         try:
             import oracle
             if oracle.have_ref():
                 t0 = time.perf_counter()
-                cpu = cpu_baseline(col, grid, dS.to_host(), args.cpu_groups)
+                cpu = cpu_baseline(col, grid, dS.to_host(), args.cpu_images, tiles, side, ref_vals)
                 log(f'cpu baseline: {cpu} ({time.perf_counter() - t0:.1f} s)')
         except Exception as e:          # the baseline is reported, not required
             log(f'cpu baseline skipped: {type(e).__name__}: {e}')
@@ -503,9 +570,12 @@ def main():
                                      f' {args.history}, factorized backprojection, T={former.T}, {former.levels} levels, '
                                      + (f'learned filters of {taps.T} taps per phase' if taps else f'{args.atten:g} dB filters')),
                            history=args.history, precision=args.precision,
-                           s_per_image=round(dt, 4), error_db=round(err, 2), error_tiles=f'{len(tiles)} x {side}^2',
+                           s_per_image=round(dt, 4), device_median_ms=round(float(np.median(step_s)) * 1e3, 2),
+                           error_db=round(err, 2), error_tiles=f'{len(tiles)} x {side}^2',
                            predicted_error_db=round(former.predicted_error_db, 2), parallelism='single GPU',
-                           host_to_host_ms=None if h2h is None else round(h2h * 1e3, 1), learned_form=learned_form),
+                           host_to_host_ms=None if h2h is None else round(h2h * 1e3, 1),
+                           host_to_host_steps=args.host_steps if h2h is not None else 0, learned_form=learned_form,
+                           exact_form=exact),
                roofline=roofline, cpu_baseline=cpu, stages=stages)
     print(json.dumps(out), flush=True)
 

@@ -1,8 +1,9 @@
 // Exact backprojection for gfx950:
 //   ExactFormer (fastsar/exact.py:404-801): per chunk of pulses, the windowed history centred and zero padded to
 //   nfft = upsample K (k_pad_window), an inverse FFT (hipFFT), the reachable bins of each profile kept (k_crop), and
-//   k_bp_tiles: per tile of TX x TY pixels and pulse the tile-centre terms in float64, per pixel the third-order
-//   range expansion in float32 and a cubic (or linear) read of the profile.
+//   k_bp_region: per region of 32 x 32 pixels the profile windows of a batch of pulses staged in LDS, the range
+//   expansion about each expansion tile's centre (terms in float64 per tile and pulse), per pixel the expansion in
+//   float32, a cubic (or linear) read from LDS and the phase's sine and cosine on the hardware's revolution units.
 //   fsh_bp_points_f64 (fastsar/bp.py:99-170 in float64): exact float64 ranges and phases onto arbitrary points,
 //   profiles in complex128; the checker for large images.
 #include "fsh_common.h"
@@ -13,7 +14,10 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace fsh {
@@ -133,132 +137,213 @@ __global__ __launch_bounds__(256) void k_to_half(const float2* __restrict__ win,
     }
 }
 
-// ------------------------------------------------------------------------------------------------ bp_tiles
+// ------------------------------------------------------------------------------------------------ region kernel
+
+// One workgroup of 256 threads forms a region of BR x BR pixels (BR 32: four pixels per thread, in rows i, i + 8,
+// i + 16, i + 24 of one column; BR 16: one). The range expansion is taken about the centre of each expansion tile
+// (TX x TY, the planner's choice) that the region overlaps; the profiles are read from LDS.
+//
+// Per batch of NB pulses:
+//   1. per pulse, the first bin of the region's window: the region centre's bin less the region's half-diagonal in
+//      bins and the cubic taps' reach, so that every pixel of the region reads inside [lo, lo + NWIN);
+//   2. the window of every pulse copied into LDS, coalesced (float16 profiles converted on the way), and per (tile,
+//      pulse) the centre terms in float64, stored as float32:
+//        alpha = e1.u, beta = e2.u (u the unit vector antenna -> tile centre), g = 1/(2r), h = 1/(2r^2),
+//        tb = the centre's bin within the window, and the phase split as
+//        ph0 + phi1 li + phi2 lj with phi1 = kcyc alpha spx, phi2 = kcyc beta spy wrapped to [-1/2, 1/2] (exact for
+//        the integer pixel offsets li, lj within the tile) and ph0 wrapped after the half-tile offsets are taken out;
+//   3. per pixel at (a, c) metres from its tile's centre along (e1, e2): du = alpha a + beta c, the third-order
+//      remainder q = (|d|^2 - du^2)(g - du h), the bin t = tb + (du + q) inv_dr, a cubic (or linear) read of the
+//      window, and the phase ph0 + phi1 li + phi2 lj + kcyc q in cycles, whose sine and cosine the hardware takes in
+//      revolutions (v_sin_f32, v_cos_f32).
+// The image accumulates over the chunks of a call (out +=), each pixel owned by one thread, in pulse order.
+// Same sums as fastsar/exact.py:71-162 (the range expansion and its interpolation), organised for LDS staging.
+// PT: profiles in global memory as float32 (0), float16 (1) or float16 pairs (2: sample m at half2 2m); with float16
+// the chunk's scale (smax / 2048) is undone once per pixel when the chunk's sum is added to the image.
+struct BpArgs {
+    const void* win;               // [n, W] cropped profiles of the chunk
+    const unsigned* smax;          // float16 profiles: the chunk's largest magnitude
+    int P, W, nfft, NB, NWIN, TB;  // pulses of the chunk, window per pulse, FFT length, pulses per window batch,
+                                   // LDS bins per window, pulses per batch of centre terms (a multiple of NB)
+    const long long* blo;          // [n] first cropped bin of each pulse (zero delay at nfft / 2)
+    const double* ant;             // [n, 3]
+    const double* ref;             // [n]
+    const double* cen;             // [ntx nty, 3] tile centres
+    int ntx, nty, nx, ny, nbx, nby;
+    int along_e1;                  // 1: regions next along e1 are formed together (e1 the more across-range axis)
+    double spx, spy, inv_dr, kcyc, reach;    // reach: the region's half-diagonal in metres
+    double e1x, e1y, e1z, e2x, e2y, e2z;
+    float2* out;
+};
 
 template <int PT>
-__device__ __forceinline__ float2 ld_prof(const void* w, size_t i)
+__device__ __forceinline__ float2 prof_at(const void* w, size_t i)
 {
     if (PT == 0) return static_cast<const float2*>(w)[i];
-    return h2f(static_cast<const half2_t*>(w)[i]);
+    const half2_t h = static_cast<const half2_t*>(w)[PT == 2 ? 2 * i : i];
+    return make_float2((float)h.x, (float)h.y);
 }
 
-// One workgroup of 256 threads: NTL tiles of TX x TY pixels (T = TX TY; NTL = max(1, 256 / T)), PX = max(1, T / 256)
-// pixels per thread. Per batch of PB = 256 / NTL pulses each thread computes one (tile, pulse) centre term in
-// float64: unit vector u, 1/(2r), 1/(2r^2), the integer and fractional profile bin and the wrapped phase. Per pixel
-// at offset d from the centre: du = d.u, del = du + (|d|^2 - du^2)(1/(2r) - du/(2r^2)), bin t = tf + del inv_dr,
-// read cubic (or linear), phase ph0 + kcyc del. The image accumulates over the chunks of a call (out +=), each pixel
-// owned by one thread, in pulse order. (fastsar/exact.py:71-162)
-// PT: the profiles' storage, 0 float32 (float2), 1 float16 (half2), 2 float16 pairs (two half2 per sample: m, m + 1).
-// With float16 the chunk's scale (smax / 2048) is undone once per pixel when the chunk's sum is added to the image.
-template <int TX, int TY, int PT>
-__global__ __launch_bounds__(256) void k_bp_tiles(const void* __restrict__ winv, const unsigned* __restrict__ smax, int P, int W,
-                                                   const long long* __restrict__ blo,
-                                                   int nfft, const double* __restrict__ ant, const double* __restrict__ ref,
-                                                   const double* __restrict__ cen, int ntx, int nty, int nx, int ny, float sx,
-                                                   float sy, float e1x, float e1y, float e1z, float e2x, float e2y, float e2z,
-                                                   double inv_dr, double kcyc, int cubic, float2* __restrict__ out)
+template <int BR, int TX, int TY, int PT, bool CUBIC>
+__device__ __forceinline__ void bp_region(const BpArgs& A)
 {
-    constexpr int T = TX * TY;
-    constexpr int NTL = T >= 256 ? 1 : 256 / T;
-    constexpr int PX = T >= 256 ? T / 256 : 1;
-    constexpr int TPT = 256 / NTL;
-    constexpr int PB = TPT;
-    __shared__ float s_ux[NTL][PB], s_uy[NTL][PB], s_uz[NTL][PB], s_ir[NTL][PB], s_i2[NTL][PB], s_tf[NTL][PB], s_ph[NTL][PB];
-    __shared__ int s_ti[NTL][PB];
-    const int tl = threadIdx.x / TPT, j = threadIdx.x - tl * TPT;
-    const int b = blockIdx.x * NTL + tl;
-    const bool bok = b < ntx * nty;
-    const int bx = bok ? b / nty : 0, by = bok ? b - bx * nty : 0;
-    const double cx = bok ? cen[3 * b] : 0.0, cy = bok ? cen[3 * b + 1] : 0.0, cz = bok ? cen[3 * b + 2] : 0.0;
-    float dx[PX], dy[PX], dz[PX], dd[PX], ar[PX], ai[PX];
-    int ix[PX], iy[PX];
+    constexpr int PXT = BR * BR / 256;                 // pixels per thread
+    constexpr int RS = 256 / BR;                       // rows between a thread's pixels
+    constexpr int SX = TX < BR ? TX : BR, SY = TY < BR ? TY : BR;
+    constexpr int NTX = BR / SX, NTY = BR / SY, NT = NTX * NTY;   // expansion tiles a region overlaps
+    constexpr int MPS = SX > RS ? (SX / RS < PXT ? SX / RS : PXT) : 1;   // consecutive pixels of a thread in one tile
+    extern __shared__ float4 smem[];
+    const int NB = A.NB, NWIN = A.NWIN, TB = A.TB;
+    float2* s_win = reinterpret_cast<float2*>(smem);                       // [NB][NWIN]
+    float4* s_geo = smem + (NB * NWIN + 1) / 2;                            // [TB][NT] alpha, beta, g, h
+    float4* s_pha = s_geo + TB * NT;                                       // [TB][NT] tb, ph0, phi1, phi2
+    int* s_lo = reinterpret_cast<int*>(s_pha + TB * NT);                   // [TB]
+
+    // regions in XCD-major order: consecutive workgroups land on different XCDs, so the regions one XCD forms are
+    // a contiguous band of the image, a band of range when the across-range axis runs fastest; the regions an XCD
+    // forms at once then lie at nearly one range and read nearly the same window of each profile from its L2
+    const int G = A.nbx * A.nby, per = (G + 7) / 8;
+    const int lg = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (lg >= G) return;
+    const int rx = A.along_e1 ? lg % A.nbx : lg / A.nby, ry = A.along_e1 ? lg / A.nbx : lg % A.nby;
+    const int i0 = rx * BR, j0 = ry * BR;
+    const int tid = threadIdx.x, col = tid % BR, row = tid / BR;
+
+    // the pixels of this thread, all in one column: tile slot, integer row offset in the tile, metres from the tile
+    // centre along e1, |d|^2; the column's offset and metres along e2 are the same for all of them
+    int slot[PXT];
+    float pa[PXT], pdd[PXT], pli[PXT], ar[PXT], ai[PXT];
+    const int gj = j0 + col, tj = gj / TY, lj = gj - tj * TY;
+    const double cm = (lj - 0.5 * (TY - 1)) * A.spy;
+    const float pc = (float)cm, plj = (float)lj;
     #pragma unroll
-    for (int m = 0; m < PX; ++m) {
-        const int q = m * TPT + j, li = q / TY, lj = q - li * TY;
-        ix[m] = bx * TX + li;
-        iy[m] = by * TY + lj;
-        const float a = (li - 0.5f * (TX - 1)) * sx, c = (lj - 0.5f * (TY - 1)) * sy;
-        dx[m] = a * e1x + c * e2x;
-        dy[m] = a * e1y + c * e2y;
-        dz[m] = a * e1z + c * e2z;
-        dd[m] = dx[m] * dx[m] + dy[m] * dy[m] + dz[m] * dz[m];
-        ar[m] = 0.f;
-        ai[m] = 0.f;
+    for (int m = 0; m < PXT; ++m) {
+        const int gi = i0 + row + RS * m;
+        const int ti = gi / TX;
+        slot[m] = ((gi - i0) / SX) * NTY + (gj - j0) / SY;
+        const int li = gi - ti * TX;
+        const double a = (li - 0.5 * (TX - 1)) * A.spx;
+        const double dx = a * A.e1x + cm * A.e2x, dy = a * A.e1y + cm * A.e2y, dz = a * A.e1z + cm * A.e2z;
+        pa[m] = (float)a;
+        pdd[m] = (float)(dx * dx + dy * dy + dz * dz);
+        pli[m] = (float)li;
+        ar[m] = ai[m] = 0.f;
     }
-    const float fdr = (float)inv_dr, fk = (float)kcyc;
-    for (int p0 = 0; p0 < P; p0 += PB) {
-        const int p = p0 + j;
-        if (bok && p < P) {
-            const double wx = cx - ant[3 * p], wy = cy - ant[3 * p + 1], wz = cz - ant[3 * p + 2];
-            const double r = sqrt(wx * wx + wy * wy + wz * wz), ir = 1.0 / r, dR = r - ref[p];
-            const double t = dR * inv_dr - (double)(blo[p] - nfft / 2), tf = floor(t), ph = kcyc * dR;
-            s_ux[tl][j] = (float)(wx * ir);
-            s_uy[tl][j] = (float)(wy * ir);
-            s_uz[tl][j] = (float)(wz * ir);
-            s_ir[tl][j] = (float)(0.5 * ir);
-            s_i2[tl][j] = (float)(0.5 * ir * ir);
-            s_ti[tl][j] = (int)tf;
-            s_tf[tl][j] = (float)(t - tf);
-            s_ph[tl][j] = (float)(ph - rint(ph));
+    // the region's centre, from the first tile it overlaps
+    const int ti0 = min(i0 / TX, A.ntx - 1), tj0 = min(j0 / TY, A.nty - 1);
+    const double* c00 = A.cen + 3 * ((size_t)ti0 * A.nty + tj0);
+    const double oa = (i0 + 0.5 * (BR - 1) - (ti0 * TX + 0.5 * (TX - 1))) * A.spx;
+    const double oc = (j0 + 0.5 * (BR - 1) - (tj0 * TY + 0.5 * (TY - 1))) * A.spy;
+    const double rcx = c00[0] + oa * A.e1x + oc * A.e2x, rcy = c00[1] + oa * A.e1y + oc * A.e2y,
+                 rcz = c00[2] + oa * A.e1z + oc * A.e2z;
+    const int half = (int)ceil(A.reach * A.inv_dr) + 3;
+    const float fdr = (float)A.inv_dr, fk = (float)A.kcyc;
+    const float tmax = (float)(CUBIC ? NWIN - 3 : NWIN - 2) + 0.5f, tmin = CUBIC ? 1.f : 0.f;
+
+    for (int pt = 0; pt < A.P; pt += TB) {
+        const int ntb = min(TB, A.P - pt);
+        __syncthreads();                                   // the previous terms and windows are read
+        // per pulse of the term batch, the first bin of the region's window
+        for (int k = tid; k < ntb; k += 256) {
+            const int p = pt + k;
+            const double wx = rcx - A.ant[3 * p], wy = rcy - A.ant[3 * p + 1], wz = rcz - A.ant[3 * p + 2];
+            const double t = (sqrt(wx * wx + wy * wy + wz * wz) - A.ref[p]) * A.inv_dr - (double)(A.blo[p] - A.nfft / 2);
+            s_lo[k] = (int)floor(t) - half;
         }
         __syncthreads();
-        const int np = min(PB, P - p0);
-        if (bok) {
-            for (int k = 0; k < np; ++k) {
-                const float ux = s_ux[tl][k], uy = s_uy[tl][k], uz = s_uz[tl][k], hir = s_ir[tl][k], hi2 = s_i2[tl][k];
-                const float tfk = s_tf[tl][k], phk = s_ph[tl][k];
-                const size_t row0 = (size_t)(p0 + k) * W;
-                const int ti = s_ti[tl][k];
+        for (int e = tid; e < ntb * NT; e += 256) {        // centre terms per (pulse, tile)
+            const int k = e / NT, s = e - k * NT, p = pt + k;
+            const int ti = min((i0 + (s / NTY) * SX) / TX, A.ntx - 1), tj = min((j0 + (s % NTY) * SY) / TY, A.nty - 1);
+            const double* cc = A.cen + 3 * ((size_t)ti * A.nty + tj);
+            const double wx = cc[0] - A.ant[3 * p], wy = cc[1] - A.ant[3 * p + 1], wz = cc[2] - A.ant[3 * p + 2];
+            const double r = sqrt(wx * wx + wy * wy + wz * wz), ir = 1.0 / r, dR = r - A.ref[p];
+            const double al = (wx * A.e1x + wy * A.e1y + wz * A.e1z) * ir, be = (wx * A.e2x + wy * A.e2y + wz * A.e2z) * ir;
+            const double tb = dR * A.inv_dr - (double)(A.blo[p] - A.nfft / 2) - (double)s_lo[k];
+            const double f1 = A.kcyc * al * A.spx, f2 = A.kcyc * be * A.spy;
+            const double ph = A.kcyc * dR - f1 * 0.5 * (TX - 1) - f2 * 0.5 * (TY - 1);
+            s_geo[k * NT + s] = make_float4((float)al, (float)be, (float)(0.5 * ir), (float)(0.5 * ir * ir));
+            s_pha[k * NT + s] = make_float4((float)tb, (float)(ph - floor(ph)), (float)(f1 - floor(f1)), (float)(f2 - floor(f2)));
+        }
+        for (int p0 = 0; p0 < ntb; p0 += NB) {
+            const int nb = min(NB, ntb - p0);
+            if (p0 > 0) __syncthreads();                   // the previous windows are read
+            for (int k = 0; k < nb; ++k) {                 // the windows, a pulse at a time across the workgroup
+                const int lo = s_lo[p0 + k];
+                const size_t rowp = (size_t)(pt + p0 + k) * A.W;
+                for (int m = tid; m < NWIN; m += 256) {
+                    const int b = lo + m;
+                    s_win[k * NWIN + m] = (b >= 0 && b < A.W) ? prof_at<PT>(A.win, rowp + b) : make_float2(0.f, 0.f);
+                }
+            }
+            __syncthreads();
+            for (int k = 0; k < nb; ++k) {
+                const float2* wrow = s_win + k * NWIN;
+                const int kt = (p0 + k) * NT;
                 #pragma unroll
-                for (int m = 0; m < PX; ++m) {
-                    const float du = dx[m] * ux + dy[m] * uy + dz[m] * uz;
-                    const float del = du + (dd[m] - du * du) * (hir - du * hi2);
-                    const float t = tfk + del * fdr, fl = floorf(t), w = t - fl;
-                    const int i = min(max(ti + (int)fl, 1), W - 3);
-                    float vre, vim;
-                    if (PT != 2 && cubic) {
-                        const float2 a0 = ld_prof<PT>(winv, row0 + i - 1), a1 = ld_prof<PT>(winv, row0 + i);
-                        const float2 a2 = ld_prof<PT>(winv, row0 + i + 1), a3 = ld_prof<PT>(winv, row0 + i + 2);
-                        const float wm1 = w - 1.f, wm2 = w - 2.f, wp1 = w + 1.f;
-                        const float c0 = -w * wm1 * wm2 * (1.f / 6.f), c1 = wp1 * wm1 * wm2 * 0.5f;
-                        const float c2 = -wp1 * w * wm2 * 0.5f, c3 = wp1 * w * wm1 * (1.f / 6.f);
-                        vre = c0 * a0.x + c1 * a1.x + c2 * a2.x + c3 * a3.x;
-                        vim = c0 * a0.y + c1 * a1.y + c2 * a2.y + c3 * a3.y;
-                    } else {
-                        float2 r0, r1;
-                        if (PT == 2) {
-                            const half2_t* hp = static_cast<const half2_t*>(winv) + 2 * (row0 + i);
-                            r0 = h2f(hp[0]);
-                            r1 = h2f(hp[1]);
+                for (int g0 = 0; g0 < PXT; g0 += MPS) {
+                    const float4 g = s_geo[kt + slot[g0]], q4 = s_pha[kt + slot[g0]];
+                    const float bc = g.y * pc;                         // the column's part of d.u
+                    const float ph0 = fmaf(q4.w, plj, q4.y);           // the column's part of the phase
+                    #pragma unroll
+                    for (int m = g0; m < g0 + MPS; ++m) {
+                        const float du = fmaf(g.x, pa[m], bc);
+                        const float q = fmaf(-du, du, pdd[m]) * fmaf(-du, g.w, g.z);
+                        // t >= tmin >= 0: truncation is the floor
+                        const float t = __builtin_amdgcn_fmed3f(fmaf(du + q, fdr, q4.x), tmin, tmax);
+                        const float w = __builtin_amdgcn_fractf(t);
+                        const int i = (int)t;
+                        float vr, vi;
+                        if (CUBIC) {
+                            // six times the cubic Lagrange read through samples i - 1 .. i + 2 at i + w:
+                            // 6 c0 = -A (w - 2), 6 c1 = 3 (A - 2)(w - 1), 6 c2 = -3 (A - 2) w, 6 c3 = A (w + 1),
+                            // A = w (w - 1); the 1/6 is taken out once per pixel at the end
+                            const float2 y0 = wrow[i - 1], y1 = wrow[i], y2 = wrow[i + 1], y3 = wrow[i + 2];
+                            const float A2 = fmaf(w, w, -w);
+                            const float k0 = A2 * (w - 2.f), k3 = fmaf(A2, w, A2);
+                            const float B3 = fmaf(A2, 3.f, -6.f);
+                            const float k1 = B3 * (w - 1.f), k2 = B3 * w;
+                            vr = fmaf(k3, y3.x, fmaf(-k2, y2.x, fmaf(k1, y1.x, -k0 * y0.x)));
+                            vi = fmaf(k3, y3.y, fmaf(-k2, y2.y, fmaf(k1, y1.y, -k0 * y0.y)));
                         } else {
-                            r0 = ld_prof<PT>(winv, row0 + i);
-                            r1 = ld_prof<PT>(winv, row0 + i + 1);
+                            const float2 y0 = wrow[i], y1 = wrow[i + 1];
+                            vr = fmaf(w, y1.x - y0.x, y0.x);
+                            vi = fmaf(w, y1.y - y0.y, y0.y);
                         }
-                        vre = fmaf(w, r1.x - r0.x, r0.x);
-                        vim = fmaf(w, r1.y - r0.y, r0.y);
+                        // the phase in cycles, reduced to [0, 1) for the revolution-unit sine and cosine
+                        const float ph = __builtin_amdgcn_fractf(fmaf(fk, q, fmaf(q4.z, pli[m], ph0)));
+                        const float sn = __builtin_amdgcn_sinf(ph), cs = __builtin_amdgcn_cosf(ph);
+                        ar[m] = fmaf(vr, cs, fmaf(-vi, sn, ar[m]));
+                        ai[m] = fmaf(vr, sn, fmaf(vi, cs, ai[m]));
                     }
-                    float ph = fmaf(fk, del, phk);
-                    ph -= rintf(ph);
-                    float s, c;
-                    sincospif(2.0f * ph, &s, &c);
-                    ar[m] = fmaf(vre, c, fmaf(-vim, s, ar[m]));
-                    ai[m] = fmaf(vre, s, fmaf(vim, c, ai[m]));
                 }
             }
         }
-        __syncthreads();
     }
-    if (!bok) return;
-    const float us = PT ? 1.f / profile_scale(smax) : 1.f;
+    const float us = (PT ? fmaxf(__uint_as_float(*A.smax), 1e-30f) / 2048.f : 1.f) * (CUBIC ? 1.f / 6.f : 1.f);
     #pragma unroll
-    for (int m = 0; m < PX; ++m)
-        if (ix[m] < nx && iy[m] < ny) {
-            const size_t o = (size_t)ix[m] * ny + iy[m];
-            float2 v = out[o];
+    for (int m = 0; m < PXT; ++m) {
+        const int gi = i0 + row + RS * m, gj = j0 + col;
+        if (gi < A.nx && gj < A.ny) {
+            const size_t o = (size_t)gi * A.ny + gj;
+            float2 v = A.out[o];
             v.x = fmaf(ar[m], us, v.x);
             v.y = fmaf(ai[m], us, v.y);
-            out[o] = v;
+            A.out[o] = v;
         }
+    }
+}
+
+// the kernels: linear reads fit six waves per SIMD in registers; cubic reads are held to five (102 VGPRs)
+template <int BR, int TX, int TY, int PT>
+__global__ __launch_bounds__(256) void k_bp_region_linear(BpArgs A)
+{
+    bp_region<BR, TX, TY, PT, false>(A);
+}
+
+template <int BR, int TX, int TY, int PT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_bp_region_cubic(BpArgs A)
+{
+    bp_region<BR, TX, TY, PT, true>(A);
 }
 
 // ------------------------------------------------------------------------------------------------ float64 points
@@ -310,6 +395,7 @@ __global__ __launch_bounds__(256) void k_bp_points_f64(const double2* __restrict
 
 struct Exact {
     fsh_exact_desc d;
+    int along_e1 = 1;                // the region kernel's order (BpArgs::along_e1)
     hipStream_t stream = nullptr;
     double* ant = nullptr;
     double* ref = nullptr;
@@ -353,40 +439,126 @@ static int upload_e(T** dst, const T* src, size_t count)
     return OK;
 }
 
-template <int PT>
-static int launch_bp_tiles_t(const Exact& e, const void* win, int n, int p0, float2* out)
+// the region kernel's launch: the region side (32, else 16 when a 32-pixel region's windows leave fewer than four
+// pulses per batch in the LDS budget), the window length and the pulses per batch
+struct BpShape {
+    int BR = 0, NWIN = 0, NB = 0, TB = 0;
+    double reach = 0.0;
+    size_t lds = 0;
+};
+
+static double region_reach(const fsh_exact_desc& d, int BR)
 {
-    const fsh_exact_desc& d = e.d;
-    const int ntiles = d.ntx * d.nty;
-    const float sx = (float)d.spx, sy = (float)d.spy;
-#define FSH_BP(TX_, TY_)                                                                                              \
-    do {                                                                                                              \
-        constexpr int NTL = (TX_ * TY_) >= 256 ? 1 : 256 / (TX_ * TY_);                                               \
-        hipLaunchKernelGGL((k_bp_tiles<TX_, TY_, PT>), dim3((ntiles + NTL - 1) / NTL), dim3(256), 0, e.stream, win, e.smax, \
-                           n, d.W, e.blo + p0, d.nfft, e.ant + 3 * (size_t)p0, e.ref + p0, e.cen, d.ntx, d.nty, d.nx, d.ny, \
-                           sx, sy, (float)d.e1[0], (float)d.e1[1], (float)d.e1[2], (float)d.e2[0], (float)d.e2[1],      \
-                           (float)d.e2[2], d.inv_dr, d.kcyc, d.cubic, out);                                            \
-    } while (0)
-    if (d.tx == 32 && d.ty == 32) FSH_BP(32, 32);
-    else if (d.tx == 16 && d.ty == 16) FSH_BP(16, 16);
-    else if (d.tx == 8 && d.ty == 16) FSH_BP(8, 16);
-    else if (d.tx == 8 && d.ty == 8) FSH_BP(8, 8);
-    else if (d.tx == 4 && d.ty == 4) FSH_BP(4, 4);
+    double r = 0.0;
+    for (int sa = -1; sa <= 1; sa += 2)
+        for (int sc = -1; sc <= 1; sc += 2) {
+            const double a = sa * 0.5 * (BR - 1) * d.spx, c = sc * 0.5 * (BR - 1) * d.spy;
+            double v2 = 0.0;
+            for (int k = 0; k < 3; ++k) v2 += (a * d.e1[k] + c * d.e2[k]) * (a * d.e1[k] + c * d.e2[k]);
+            r = std::max(r, std::sqrt(v2));
+        }
+    return r * (1.0 + 1e-9) + 1e-9;
+}
+
+static bool bp_shape(const fsh_exact_desc& d, BpShape& sh)
+{
+    size_t budget = 32 * 1024;
+    if (const char* e = getenv("FSH_BP_LDS_KB")) budget = (size_t)std::max(8, std::min(64, atoi(e))) * 1024;   // sweeps
+    for (int BR : {32, 16}) {
+        const int sx = std::min(d.tx, BR), sy = std::min(d.ty, BR);
+        const int NT = (BR / sx) * (BR / sy);
+        sh.BR = BR;
+        sh.reach = region_reach(d, BR);
+        const double half = std::ceil(sh.reach * d.inv_dr) + 3.0;
+        if (half > 1e6) return false;
+        sh.NWIN = 2 * (int)half + 2;
+        // windows of NB pulses; centre terms of TB = NB * max(1, 256 / (NB NT)) pulses (up to 256 (pulse, tile) pairs)
+        for (int NB = 32; NB >= (BR == 32 ? 4 : 1); --NB) {
+            const int TB = NB * std::max(1, 256 / (NB * NT));
+            const size_t lds = ((size_t)NB * sh.NWIN + 1) / 2 * 16 + (size_t)2 * TB * NT * 16 + (size_t)TB * 4;
+            if (lds <= budget) {
+                sh.NB = NB;
+                sh.TB = TB;
+                sh.lds = lds;
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+template <int BR, int TX, int TY, int PT>
+static void launch_region(const BpArgs& a, bool cubic, size_t lds, hipStream_t s)
+{
+    const int G = a.nbx * a.nby;
+    const dim3 grid((unsigned)(8 * ((G + 7) / 8)));
+    if (cubic) hipLaunchKernelGGL((k_bp_region_cubic<BR, TX, TY, PT>), grid, dim3(256), lds, s, a);
+    else hipLaunchKernelGGL((k_bp_region_linear<BR, TX, TY, PT>), grid, dim3(256), lds, s, a);
+}
+
+template <int BR, int PT>
+static int launch_region_tile(const fsh_exact_desc& d, const BpArgs& a, size_t lds, hipStream_t s)
+{
+    if (d.tx == 32 && d.ty == 32) launch_region<BR, 32, 32, PT>(a, d.cubic, lds, s);
+    else if (d.tx == 16 && d.ty == 16) launch_region<BR, 16, 16, PT>(a, d.cubic, lds, s);
+    else if (d.tx == 8 && d.ty == 16) launch_region<BR, 8, 16, PT>(a, d.cubic, lds, s);
+    else if (d.tx == 8 && d.ty == 8) launch_region<BR, 8, 8, PT>(a, d.cubic, lds, s);
+    else if (d.tx == 4 && d.ty == 4) launch_region<BR, 4, 4, PT>(a, d.cubic, lds, s);
     else {
-        set_error("exact: unsupported tile");
+        set_error("exact: unsupported tile " + std::to_string(d.tx) + " x " + std::to_string(d.ty));
         return E_INVALID;
     }
-#undef FSH_BP
     FSH_TRY(hipGetLastError());
     return OK;
 }
 
-static int launch_bp_tiles(const Exact& e, int n, int p0, float2* out)
+static int launch_bp(const Exact& e, int n, int p0, float2* out)
 {
-    switch (e.d.profiles) {
-    case 1: return launch_bp_tiles_t<1>(e, e.winh, n, p0, out);
-    case 2: return launch_bp_tiles_t<2>(e, e.winh, n, p0, out);
-    default: return launch_bp_tiles_t<0>(e, e.win, n, p0, out);
+    const fsh_exact_desc& d = e.d;
+    BpShape sh;
+    if (!bp_shape(d, sh)) {
+        set_error("exact: a pixel region's range window does not fit in LDS (use a smaller upsample or pixel spacing)");
+        return E_INVALID;
+    }
+    BpArgs a;
+    a.win = d.profiles ? e.winh : (const void*)e.win;
+    a.smax = e.smax;
+    a.P = n;
+    a.W = d.W;
+    a.nfft = d.nfft;
+    a.NB = sh.NB;
+    a.NWIN = sh.NWIN;
+    a.TB = sh.TB;
+    a.blo = e.blo + p0;
+    a.ant = e.ant + 3 * (size_t)p0;
+    a.ref = e.ref + p0;
+    a.cen = e.cen;
+    a.ntx = d.ntx;
+    a.nty = d.nty;
+    a.nx = d.nx;
+    a.ny = d.ny;
+    a.nbx = (d.nx + sh.BR - 1) / sh.BR;
+    a.nby = (d.ny + sh.BR - 1) / sh.BR;
+    a.along_e1 = e.along_e1;
+    a.spx = d.spx;
+    a.spy = d.spy;
+    a.inv_dr = d.inv_dr;
+    a.kcyc = d.kcyc;
+    a.reach = sh.reach;
+    a.e1x = d.e1[0]; a.e1y = d.e1[1]; a.e1z = d.e1[2];
+    a.e2x = d.e2[0]; a.e2y = d.e2[1]; a.e2z = d.e2[2];
+    a.out = out;
+    if (sh.BR == 32) {
+        switch (d.profiles) {
+        case 1: return launch_region_tile<32, 1>(d, a, sh.lds, e.stream);
+        case 2: return launch_region_tile<32, 2>(d, a, sh.lds, e.stream);
+        default: return launch_region_tile<32, 0>(d, a, sh.lds, e.stream);
+        }
+    }
+    switch (d.profiles) {
+    case 1: return launch_region_tile<16, 1>(d, a, sh.lds, e.stream);
+    case 2: return launch_region_tile<16, 2>(d, a, sh.lds, e.stream);
+    default: return launch_region_tile<16, 0>(d, a, sh.lds, e.stream);
     }
 }
 
@@ -409,6 +581,19 @@ int fsh_exact_create(const fsh_exact_desc* desc, void** handle)
     }
     Exact* e = new Exact();
     e->d = d;
+    {
+        // the line of sight from the track's mean to the grid's mean, against the grid's axes: the axis it has the
+        // smaller part along is the across-range one
+        double m[3] = {0, 0, 0};
+        const long nt = (long)d.ntx * d.nty;
+        for (long i = 0; i < nt; ++i)
+            for (int k = 0; k < 3; ++k) m[k] += d.cen[3 * i + k] / (double)nt;
+        for (int p = 0; p < d.P; ++p)
+            for (int k = 0; k < 3; ++k) m[k] -= d.ant[3 * (size_t)p + k] / (double)d.P;
+        const double c1 = std::fabs(m[0] * d.e1[0] + m[1] * d.e1[1] + m[2] * d.e1[2]);
+        const double c2 = std::fabs(m[0] * d.e2[0] + m[1] * d.e2[1] + m[2] * d.e2[2]);
+        e->along_e1 = c1 <= c2 ? 1 : 0;
+    }
     int st = OK;
     auto fail = [&](int s) { delete e; return s; };
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) { set_error("exact: stream"); return fail(E_RUNTIME); }
@@ -494,7 +679,7 @@ static int exact_form_impl(Exact* e, const void* S, int dtype, const float* puls
         else if (d.profiles == 2)
             hipLaunchKernelGGL(k_to_half<true>, gw, dim3(256), 0, e->stream, e->win, n, d.W, e->smax, (half2_t*)e->winh);
         FSH_TRY(hipGetLastError());
-        if ((st = launch_bp_tiles(*e, n, p0, e->image))) return st;
+        if ((st = launch_bp(*e, n, p0, e->image))) return st;
     }
     int badh = INT_MAX;
     FSH_TRY(hipMemcpyAsync(&badh, e->bad, sizeof(int), hipMemcpyDeviceToHost, e->stream));

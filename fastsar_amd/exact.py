@@ -22,7 +22,7 @@ import numpy as np
 from . import _hip
 from .plan import C
 
-TILES = ((32, 32), (16, 16), (8, 16), (8, 8))     # candidate tiles, largest first (fastsar/exact.py:366-368)
+TILES = ((32, 32), (16, 16), (8, 16), (8, 8), (4, 4))     # candidate expansion tiles, largest first
 PHASE_LIMIT = 3e-4                                 # rad (fastsar/exact.py:369)
 
 
@@ -68,11 +68,14 @@ def plane_points(nx, ny, spx, spy, e1=(1.0, 0.0, 0.0), e2=(0.0, 1.0, 0.0), heigh
 
 
 def _integer(v, name, least):
-    try:
-        ok = not isinstance(v, (bool, np.bool_)) and np.isscalar(v) and np.isfinite(v) and int(v) == v and v >= least
-    except (TypeError, ValueError):
-        ok = False
-    if not ok:
+    """v as an int when it is an integral finite number of at least `least`, else ValueError."""
+    bad = isinstance(v, (bool, np.bool_)) or not np.isscalar(v)
+    if not bad:
+        try:
+            bad = not (np.isfinite(v) and int(v) == v and v >= least)
+        except (TypeError, ValueError):
+            bad = True
+    if bad:
         raise ValueError(f'{name} must be an integer of at least {least}, got {v!r}')
     return int(v)
 
@@ -85,21 +88,112 @@ def _number(v):
 
 
 def _distance_to_rectangle(ant, x0, e1, e2, a0, a1, b0, b1):
-    """Per pulse the distance from ant [P, 3] to the rectangle x0 + a e1 + b e2 (a0 <= a <= a1, b0 <= b <= b1): the
-    antenna projected onto the plane and clamped into the rectangle, the exact minimum of the convex distance.
-    (fastsar/exact.py:389-396)"""
-    v = ant - x0
-    a = np.clip(v @ e1, a0, a1)
-    b = np.clip(v @ e2, b0, b1)
-    return np.linalg.norm(v - a[:, None] * e1 - b[:, None] * e2, axis=1)
+    """Per pulse the least distance from ant [P, 3] to the plane rectangle x0 + a e1 + b e2 (a in [a0, a1], b in
+    [b0, b1]; e1, e2 orthonormal). The distance to a point of the plane is convex in (a, b) and smallest at the foot
+    of the perpendicular, so clamping the foot's coordinates into the rectangle gives the minimum."""
+    rel = ant - x0
+    foot_a = np.clip(rel @ e1, a0, a1)
+    foot_b = np.clip(rel @ e2, b0, b1)
+    return np.linalg.norm(rel - foot_a[:, None] * e1 - foot_b[:, None] * e2, axis=1)
+
+
+def _farthest_corner(ant, x0, e1, e2, a0, a1, b0, b1):
+    """Per pulse the largest distance to the same rectangle: a convex function is largest at a vertex."""
+    corners = np.array([x0 + a * e1 + b * e2 for a in (a0, a1) for b in (b0, b1)])
+    return np.linalg.norm(corners[None] - ant[:, None], axis=2).max(1)
+
+
+def expansion_tile(rmin, kmax, spx, spy, tiles=TILES, limit=PHASE_LIMIT):
+    """The expansion tile: about each tile centre the range to a pixel at offset d is a series in |d| / r kept to third
+    order, whose remainder is at most h^4 / (2 r^3) metres (h the tile's half-diagonal, r the least distance from the
+    antenna to a tile centre, at least rmin - h), a phase of kmax = 4 pi f_max / c times that. Tiles are tried largest
+    first; one counts only where r >= 4 h (the series is not used closer). -> (tx, ty, remainder in metres, phase in
+    rad) of the first tile within `limit`, else of the smallest usable one; None when no tile is usable."""
+    usable = []
+    for tx, ty in tiles:
+        h = 0.5 * np.hypot((tx - 1) * spx, (ty - 1) * spy)
+        r = rmin - h
+        if r >= 4 * h:
+            rem = h ** 4 / (2 * r ** 3)
+            usable.append((tx, ty, rem, kmax * rem))
+    within = [u for u in usable if u[3] <= limit]
+    return within[0] if within else (usable[-1] if usable else None)
+
+
+class ExactPlan:
+    """The host plan of an exact former, in float64: the expansion tile (tx, ty) and its predicted phase error, the
+    tile centres cen [ntx nty, 3] (tile (i, j) at pixel (i tx + (tx - 1)/2, j ty + (ty - 1)/2) of the grid), and the
+    range bins each pulse reaches: profiles of nfft = 2^ceil(log2(upsample K)) bins (zero delay at bin nfft/2, a bin
+    1 / inv_dr metres), of which the W from blo[p] are kept. The bins cover the grid with one tile of margin on every
+    side (the last tiles may hang past the grid), widened by the expansion's remainder and the interpolation's taps.
+    Decisions as fastsar/exact.py:415-540 makes them."""
+
+    def __init__(self, ant, ref, fmin, df, K, nx, ny, spx, spy, e1, e2, center, upsample, tiles=TILES):
+        self.nfft = 1 << int(np.ceil(np.log2(upsample * K)))
+        self.inv_dr = 2.0 * df * self.nfft / C
+        self.kcyc = 2.0 * (fmin + (K // 2) * df) / C
+        x0 = center + (-nx / 2.0) * spx * e1 + (-ny / 2.0) * spy * e2          # the centre of pixel (0, 0)
+        ga, gb = (nx - 1) * spx, (ny - 1) * spy                               # the grid's extent along e1, e2
+        rmin = float(_distance_to_rectangle(ant, x0, e1, e2, 0.0, ga, 0.0, gb).min())
+        kmax = 4 * np.pi * (fmin + (K - 1) * df) / C
+        pick = expansion_tile(rmin, kmax, spx, spy, tiles)
+        if pick is None:
+            raise ValueError(f'the grid comes within {rmin:.1f} m of the antenna, too close for the tiled range expansion '
+                             f'of ExactFormer; use backproject on plane_points of the grid')
+        self.tx, self.ty, self.expansion_m, self.phase = pick
+        self.rmin = rmin
+        self.predicted_error_db = float(20 * np.log10(max(self.phase, 1e-30)))
+        self.ntx, self.nty = -(-nx // self.tx), -(-ny // self.ty)
+        ci = (np.arange(self.ntx) * self.tx + 0.5 * (self.tx - 1))[:, None, None]
+        cj = (np.arange(self.nty) * self.ty + 0.5 * (self.ty - 1))[None, :, None]
+        self.cen = np.ascontiguousarray((x0 + ci * spx * e1 + cj * spy * e2).reshape(-1, 3))
+        # reachable bins: the grid and a tile of margin, its nearest point and farthest corner per pulse
+        ma, mb = self.tx * spx, self.ty * spy
+        near = _distance_to_rectangle(ant, x0, e1, e2, -ma, ga + ma, -mb, gb + mb)
+        far = _farthest_corner(ant, x0, e1, e2, -ma, ga + ma, -mb, gb + mb)
+        self.slack = 4 + int(np.ceil(self.expansion_m * self.inv_dr))
+        zero = self.nfft // 2
+        self.blo = np.ascontiguousarray(np.floor((near - ref) * self.inv_dr).astype(np.int64) + zero - self.slack)
+        last = np.ceil((far - ref) * self.inv_dr).astype(np.int64) + zero + self.slack
+        self.W = int((last - self.blo).max()) + 4
+        self.lo = (self.blo - zero).astype(np.int32)
+        # the grid itself (no margin) must lie within one period of the profiles, c / (2 df)
+        near0 = _distance_to_rectangle(ant, x0, e1, e2, 0.0, ga, 0.0, gb)
+        far0 = _farthest_corner(ant, x0, e1, e2, 0.0, ga, 0.0, gb)
+        self.need = int((np.ceil(far0 * self.inv_dr) - np.floor(near0 * self.inv_dr)).max()) + 2 * self.slack + 4
+        self.span_m = float((far0 - near0).max())
+
+
+def _point(v, what):
+    """A finite 3-vector in metres, else ValueError."""
+    try:
+        p = np.asarray(v, np.float64)
+    except (TypeError, ValueError):
+        p = None
+    if p is None or p.shape != (3,) or not np.isfinite(p).all():
+        raise ValueError(f'{what} must be a point (a finite 3-vector) in meters, got {v!r}')
+    return p
+
+
+def _ranges(v, P):
+    """One finite range per pulse in metres, else ValueError."""
+    try:
+        r = np.asarray(v, np.float64)
+    except (TypeError, ValueError):
+        raise ValueError('ref must be a range per pulse in meters') from None
+    if r.shape != (P,):
+        raise ValueError(f'ref must hold one range per pulse, shape ({P},), got {r.shape}')
+    if not np.isfinite(r).all():
+        raise ValueError(f'ref has non-finite values (NaN or inf), the first in pulse {int(np.argmax(~np.isfinite(r)))}')
+    return r
 
 
 class ExactFormer:
     """Exact backprojection onto the grid of form_image, set up once for a collection geometry. former(S) ->
     complex64 image [nx, ny]. interp 'cubic' (default) or 'linear'; upsample the range oversampling (default 8);
     chunk the pulses range compressed at a time; ref [P] the reference ranges (default |ant|); center the grid's
-    centre (default the origin). former.tile and former.predicted_error_db as in fastsar/exact.py:404-540. A former
-    must not be called from two threads at once."""
+    centre (default the origin). former.tile (the expansion tile) and former.predicted_error_db as the reference's
+    (fastsar/exact.py:404-540). A former must not be called from two threads at once."""
 
     def __init__(self, ant, fmin, df, K, nx, ny, spx, spy, e1=(1.0, 0.0, 0.0), e2=(0.0, 1.0, 0.0), *, backend='auto',
                  window=True, interp='cubic', upsample=None, ref=None, center=None, chunk=1024, profiles='auto'):
@@ -110,107 +204,46 @@ class ExactFormer:
             raise ValueError(f"profiles must be 'auto', 'float32', 'float16' or 'float16-pair', got {profiles!r}")
         if profiles == 'float16-pair' and interp != 'linear':
             raise ValueError("profiles='float16-pair' applies to interp='linear'")
-        # 'auto' is float32 here. The reference picks float16 for linear reads on grids of 2^25 pixels and more,
-        # where its pixel gather dominates (fastsar/exact.py:420-430); on gfx950 the gather is served from cache and
-        # the kernel is bound by its arithmetic, to which the conversions add: a 4096^2 grid from 2048 pulses forms
-        # in 40.5 ms with float32 profiles and in 52.0 (float16) or 50.4 ms (pairs) with float16 ones.
+        # 'auto' keeps float32 profiles on every grid: the region kernel reads its windows from LDS, so float16
+        # storage saves only the staging reads and adds the conversions
         self.profiles = 'float32' if profiles == 'auto' else profiles
         self.ant = _check_positions(ant)
         self.P = len(self.ant)
         self.K = _integer(K, 'K (frequency samples per pulse)', 2)
-        if not (np.isfinite(_number(fmin)) and np.isfinite(_number(df)) and _number(df) > 0):
+        fmin_v, df_v = _number(fmin), _number(df)
+        if not (np.isfinite(fmin_v) and np.isfinite(df_v) and df_v > 0):
             raise ValueError(f'fmin must be a finite frequency and df a positive frequency spacing in Hz, got {fmin} and {df}')
-        fmin, df = float(fmin), float(df)
         nx, ny, e1, e2 = _check_grid(nx, ny, spx, spy, e1, e2)
         self.nx, self.ny, self.spx, self.spy = int(nx), int(ny), float(spx), float(spy)
         self.e1, self.e2 = e1, e2
-        if center is None:
-            self.center = np.zeros(3)
-        else:
-            try:
-                self.center = np.asarray(center, np.float64)
-            except (TypeError, ValueError):
-                raise ValueError(f'center must be a point (a finite 3-vector) in meters, got {center!r}') from None
-            if self.center.shape != (3,) or not np.isfinite(self.center).all():
-                raise ValueError(f'center must be a point (a finite 3-vector) in meters, got {center!r}')
-        if ref is None:
-            self.ref = np.linalg.norm(self.ant, axis=1)
-        else:
-            try:
-                self.ref = np.asarray(ref, np.float64)
-            except (TypeError, ValueError):
-                raise ValueError('ref must be a range per pulse in meters') from None
-            if self.ref.shape != (self.P,):
-                raise ValueError(f'ref must hold one range per pulse, shape ({self.P},), got {self.ref.shape}')
-            if not np.isfinite(self.ref).all():
-                raise ValueError(f'ref has non-finite values (NaN or inf), the first in pulse {int(np.argmax(~np.isfinite(self.ref)))}')
+        self.center = np.zeros(3) if center is None else _point(center, 'center')
+        self.ref = np.linalg.norm(self.ant, axis=1) if ref is None else _ranges(ref, self.P)
         self.cubic = interp == 'cubic'
         self.upsample = 8 if upsample is None else _integer(upsample, 'upsample', 1)
         self.chunk = _integer(chunk, 'chunk', 1)
         self.backend = _backend(backend)
-        self.fmin, self.df, self.window = fmin, df, window
-        self.nfft = 1 << int(np.ceil(np.log2(self.upsample * self.K)))
-        self.inv_dr = 2.0 * df * self.nfft / C
-        self.kcyc = 2.0 * (fmin + (self.K // 2) * df) / C
+        self.fmin, self.df, self.window = fmin_v, df_v, window
         self.wp, self.wk = _window(self.P, self.K) if window else (None, None)
-        self._choose_tile()
-        self.ntx, self.nty = -(-nx // self.tx), -(-ny // self.ty)
-        x0 = self.center + (-nx / 2.0) * spx * self.e1 + (-ny / 2.0) * spy * self.e2
-        bi = (np.arange(self.ntx) * self.tx + 0.5 * (self.tx - 1))[:, None, None]
-        bj = (np.arange(self.nty) * self.ty + 0.5 * (self.ty - 1))[None, :, None]
-        self.cen = np.ascontiguousarray((x0 + bi * spx * self.e1 + bj * spy * self.e2).reshape(-1, 3))
-        # the bins each pulse can reach: the grid with a tile of margin, nearest by projection and clamping, farthest
-        # at a corner; then the expansion's error and the interpolation's taps (fastsar/exact.py:478-507)
-        mx, my = self.tx * spx, self.ty * spy
-        a0, a1, b0, b1 = -mx, (nx - 1) * spx + mx, -my, (ny - 1) * spy + my
-        near = _distance_to_rectangle(self.ant, x0, self.e1, self.e2, a0, a1, b0, b1)
-        corners = np.array([x0 + a * self.e1 + b * self.e2 for a in (a0, a1) for b in (b0, b1)])
-        far = np.linalg.norm(corners[None] - self.ant[:, None], axis=2).max(1)
-        slack = 4 + int(np.ceil(self._expansion_m * self.inv_dr))
-        blo = np.floor((near - self.ref) * self.inv_dr).astype(np.int64) + self.nfft // 2 - slack
-        bhi = np.ceil((far - self.ref) * self.inv_dr).astype(np.int64) + self.nfft // 2 + slack
-        self.W = int((bhi - blo).max()) + 4
-        near0 = _distance_to_rectangle(self.ant, x0, self.e1, self.e2, 0.0, (nx - 1) * spx, 0.0, (ny - 1) * spy)
-        far0 = np.linalg.norm(np.array([x0 + a * self.e1 + b * self.e2 for a in (0.0, (nx - 1) * spx)
-                                         for b in (0.0, (ny - 1) * spy)])[None] - self.ant[:, None], axis=2).max(1)
-        need = int((np.ceil(far0 * self.inv_dr) - np.floor(near0 * self.inv_dr)).max()) + 2 * slack + 4
-        if need > self.nfft:
-            raise ValueError(f'the grid spans up to {float((far0 - near0).max()):.1f} m of range from one pulse, more than '
-                             f'this collection resolves without ambiguity (c / (2 df) = {C / (2 * df):.1f} m less the '
-                             f'interpolation margin: the grid needs {need} of the {self.nfft} range bins at '
+        plan = ExactPlan(self.ant, self.ref, fmin_v, df_v, self.K, self.nx, self.ny, self.spx, self.spy, self.e1, self.e2,
+                         self.center, self.upsample)
+        self.plan = plan
+        self.nfft, self.inv_dr, self.kcyc = plan.nfft, plan.inv_dr, plan.kcyc
+        self.tx, self.ty, self.tile = plan.tx, plan.ty, (plan.tx, plan.ty)
+        self._expansion_m, self.predicted_error_db = plan.expansion_m, plan.predicted_error_db
+        self.ntx, self.nty, self.cen = plan.ntx, plan.nty, plan.cen
+        self.blo, self.lo, self.W = plan.blo, plan.lo, plan.W
+        if plan.phase > PHASE_LIMIT:
+            warnings.warn(f'ExactFormer: at {plan.rmin:.0f} m from the antenna, even {self.tx} by {self.ty} pixel tiles leave '
+                          f'a predicted phase error of {plan.phase:.1e} rad ({self.predicted_error_db:.1f} dB) from the range '
+                          'expansion; backproject on plane_points of the grid is exact at any range',
+                          UserWarning, stacklevel=2)
+        if plan.need > self.nfft:
+            raise ValueError(f'the grid spans up to {plan.span_m:.1f} m of range from one pulse, more than '
+                             f'this collection resolves without ambiguity (c / (2 df) = {C / (2 * df_v):.1f} m less the '
+                             f'interpolation margin: the grid needs {plan.need} of the {self.nfft} range bins at '
                              f'upsample={self.upsample}); form a smaller grid (or several) or use samples spaced more '
                              'finely in frequency')
-        self.blo = np.ascontiguousarray(blo, np.int64)
-        self.lo = (blo - self.nfft // 2).astype(np.int32)
         self._h = None
-
-    def _choose_tile(self):
-        """The largest tile whose predicted phase error stays below PHASE_LIMIT. (fastsar/exact.py:511-540)"""
-        nx, ny, spx, spy = self.nx, self.ny, self.spx, self.spy
-        x0 = self.center + (-nx / 2.0) * spx * self.e1 + (-ny / 2.0) * spy * self.e2
-        rmin = float(_distance_to_rectangle(self.ant, x0, self.e1, self.e2, 0.0, (nx - 1) * spx, 0.0, (ny - 1) * spy).min())
-        kmax = 4 * np.pi * (self.fmin + (self.K - 1) * self.df) / C
-        best = None
-        for tx, ty in TILES:
-            h = 0.5 * np.hypot((tx - 1) * spx, (ty - 1) * spy)
-            r = rmin - h
-            if r < 4 * h:
-                continue
-            err = h ** 4 / (2 * r ** 3)
-            best = (tx, ty, err, kmax * err)
-            if kmax * err <= PHASE_LIMIT:
-                break
-        if best is None:
-            raise ValueError(f'the grid comes within {rmin:.1f} m of the antenna, too close for the tiled range expansion '
-                             f'of ExactFormer; use backproject on plane_points of the grid')
-        self.tx, self.ty, self._expansion_m, phase = best
-        self.tile = (self.tx, self.ty)
-        self.predicted_error_db = float(20 * np.log10(max(phase, 1e-30)))
-        if phase > PHASE_LIMIT:
-            warnings.warn(f'ExactFormer: at {rmin:.0f} m from the antenna, even {self.tx} by {self.ty} pixel tiles leave a '
-                          f'predicted phase error of {phase:.1e} rad ({self.predicted_error_db:.1f} dB) from the range '
-                          'expansion; backproject on plane_points of the grid is exact at any range',
-                          UserWarning, stacklevel=3)
 
     def _handle(self):
         if self._h is None:

@@ -58,6 +58,27 @@ def test_exact_former_vs_float64_tiles(scene):
     assert e < -70, e
 
 
+@pytest.mark.parametrize('r0,upsample,interp', [(250.0, 8, 'cubic'), (20e3, 64, 'cubic'), (20e3, 64, 'linear')],
+                         ids=['250m-4x4-tiles', '20km-upsample64-16px-regions', '20km-upsample64-linear'])
+def test_exact_region_shapes(r0, upsample, interp):
+    """the region kernel's other shapes against the float64 tiled restatement on the same host plan: 4 x 4 expansion
+    tiles near the antenna (16 tiles per 32-pixel region), and windows long enough (upsample 64) that the regions
+    shrink to 16 x 16 pixels"""
+    col, _, _, S = _scene(r0)
+    f = ExactFormer(col.ant, col.fmin, col.df, S.shape[1], **GRID, upsample=upsample, interp=interp)
+    print(f'exact former at {r0:.0f} m, upsample {upsample}, {interp}: tile {f.tile}, W {f.W}', flush=True)
+    if r0 < 1e3:
+        assert f.tile == (4, 4)
+    wp, wk = oracle.window(*S.shape)
+    rc = oracle.range_compress(S * wp[:, None] * wk[None, :], f.nfft)
+    win = np.take_along_axis(rc, (f.blo[:, None] + np.arange(f.W)[None, :]) % f.nfft, axis=1)
+    want = oracle.exact_tiles(win, f.lo, col.ant, np.linalg.norm(col.ant, axis=1), f.cen, f.ntx, f.nty, f.tx, f.ty,
+                              GRID['nx'], GRID['ny'], 0.5, 0.5, GRID['e1'], GRID['e2'], f.inv_dr, f.kcyc, interp == 'cubic')
+    e = oracle.plain_db(f(S), want)
+    print(f'  {e:.1f} dB against the float64 tiles')
+    assert e < -70, e
+
+
 def test_exact_former_chunks_and_device_io(scene):
     """the image does not depend on the pulse chunk, and device input/output give the host result"""
     _, col, S, _ = scene

@@ -1,8 +1,8 @@
 // Factorized backprojection kernels for gfx950 (CDNA4), float32 arithmetic with float64 phase seeds.
 //
 //   k_prepare       history times the Taylor windows (optional), scan for non-finite samples
-//   k_rot_fir_k     rotate each parent row by every child's phase ramp, decimate along frequency
-//                   (the stage of fastsar/ffbp_cpu.cpp:147-255 and fastsar/ffbp_cuda.py:48-187)
+//   k_rot_fir_k     decimate along frequency with every child's phase ramp folded into complex taps (VALU form;
+//                   the stage of fastsar/ffbp_cpu.cpp:147-255; the matrix-core forms are in range_mfma.hip)
 //   k_fir_p         decimate along pulses (fastsar/ffbp_cpu.cpp:257-293, fastsar/ffbp_cuda.py:190-231)
 //   k_device_phases child rotations of levels >= 2 in float64 (fastsar/ffbp2.py:578-590)
 //   k_final_geom    per tile and final subaperture: unit vector along e1, e2; per tile the aperture mean
@@ -57,71 +57,52 @@ __global__ __launch_bounds__(256) void k_prepare(const float2* __restrict__ S, i
 
 // ------------------------------------------------------------------------------------------------ rot_fir_k
 
-// grid (ceil(Ko / JB), ceil(P / PB), Np), block 256. Dynamic LDS: the raw window [PB][W] (float2), the window
-// rotated by the current child's ramp as two planes [PB][WS] (WS = W | 1), the taps [L]; W = D (JB - 1) + L.
-// A raw window is loaded once and serves every child. The ramp of row p, child c: phase c0 + (k - kc) sl cycles,
-// seeded exactly in float64 for each lane (samples l, l + 32, ...) and advanced by a float32 complex recurrence
-// 32 samples apart (at most W / 32 steps).
+// The rotation folded into the taps: with k_j = D j - pl,
+//   y_j = sum_r taps[r] x[k_j + r] e^{2 pi i (c0 + (k_j + r - kc) sl)} = e^{2 pi i (c0 + (k_j - kc) sl)} sum_r h_r x[k_j + r],
+// h_r = taps[r] e^{2 pi i r sl}: per (row, child) L complex taps formed in float64 and one phasor per output, so no
+// rotated copy of the window is made. grid (ceil(Ko / JB), ceil(P / PB), Np), block 256. Dynamic LDS: the window of
+// the block's rows [PB][W] (float2, W = D (JB - 1) + L, loaded once for every child) and the complex taps [PB][L].
 __global__ __launch_bounds__(256) void k_rot_fir_k(const float2* __restrict__ x, int P, int K, const float* __restrict__ c0,
                                                     const float* __restrict__ sl, int Cn, const float* __restrict__ taps, int L,
                                                     int pl, int D, int Ko, double kc, int JB, int PB, float2* __restrict__ y)
 {
-    extern __shared__ float smem[];
+    extern __shared__ float2 lds2[];
     const int W = D * (JB - 1) + L;
-    const int WS = W | 1;
-    float2* raw = reinterpret_cast<float2*>(smem);
-    float* zr = smem + 2 * PB * W;
-    float* zi = zr + PB * WS;
-    float* tp = zi + PB * WS;
+    float2* win = lds2;                                    // [PB][W]
+    float2* h = lds2 + (size_t)PB * W;                     // [PB][L]
     const int tid = threadIdx.x;
     const int j0 = blockIdx.x * JB, p0 = blockIdx.y * PB, n = blockIdx.z;
-    const int kstart = D * j0 - pl;
-    for (int r = tid; r < L; r += 256) tp[r] = taps[r];
+    const int first = D * j0 - pl;                         // the window's first sample
     const float2* xn = x + (size_t)n * P * K;
-    for (int idx = tid; idx < PB * W; idx += 256) {
-        const int row = idx / W, w = idx - row * W;
-        const int p = p0 + row, k = kstart + w;
-        raw[idx] = (p < P && k >= 0 && k < K) ? xn[(size_t)p * K + k] : make_float2(0.f, 0.f);
+    for (int e = tid; e < PB * W; e += 256) {
+        const int rr = e / W, k = first + (e - rr * W), p = p0 + rr;
+        win[e] = (p < P && k >= 0 && k < K) ? xn[(size_t)p * K + k] : make_float2(0.f, 0.f);
     }
-    __syncthreads();
-    const int rl = tid & 31, rrow = tid >> 5;
     for (int c = 0; c < Cn; ++c) {
-        for (int row = rrow; row < PB; row += 8) {
-            const int p = p0 + row;
-            double a = 0.0, b = 0.0;
-            if (p < P) {
-                a = (double)c0[((size_t)n * Cn + c) * P + p];
-                b = (double)sl[((size_t)n * Cn + c) * P + p];
-            }
-            float wr, wi, sr, si;
-            cis_cyc(fma((double)(kstart + rl) - kc, b, a), wr, wi);
-            cis_cyc(32.0 * b, sr, si);
-            for (int w = rl; w < W; w += 32) {
-                const float2 v = raw[row * W + w];
-                zr[row * WS + w] = fmaf(v.x, wr, -(v.y * wi));
-                zi[row * WS + w] = fmaf(v.x, wi, v.y * wr);
-                const float nr = fmaf(wr, sr, -(wi * si));
-                wi = fmaf(wr, si, wi * sr);
-                wr = nr;
-            }
+        const size_t rc = ((size_t)n * Cn + c) * P;
+        __syncthreads();                                   // the window is in; the previous child's taps are read
+        for (int e = tid; e < PB * L; e += 256) {
+            const int rr = e / L, r = e - rr * L, p = min(p0 + rr, P - 1);
+            float cs, sn;
+            cis_cyc((double)r * (double)sl[rc + p], cs, sn);
+            h[e] = make_float2(taps[r] * cs, taps[r] * sn);
         }
         __syncthreads();
         for (int o = tid; o < PB * JB; o += 256) {
-            const int pp = o / JB, jj = o - pp * JB;
-            const int p = p0 + pp, j = j0 + jj;
-            if (p < P && j < Ko) {
-                const float* rr = zr + pp * WS + D * jj;
-                const float* ri = zi + pp * WS + D * jj;
-                float ar = 0.f, ai = 0.f;
-                for (int r = 0; r < L; ++r) {
-                    const float t = tp[r];
-                    ar = fmaf(t, rr[r], ar);
-                    ai = fmaf(t, ri[r], ai);
-                }
-                y[(((size_t)n * Cn + c) * P + p) * Ko + j] = make_float2(ar, ai);
+            const int rr = o / JB, jj = o - rr * JB, p = p0 + rr, j = j0 + jj;
+            if (p >= P || j >= Ko) continue;
+            const float2* xs = win + (size_t)rr * W + D * jj;
+            const float2* hr = h + (size_t)rr * L;
+            float ar = 0.f, ai = 0.f;
+            for (int r = 0; r < L; ++r) {
+                const float2 v = xs[r], t = hr[r];
+                ar = fmaf(t.x, v.x, fmaf(-t.y, v.y, ar));
+                ai = fmaf(t.x, v.y, fmaf(t.y, v.x, ai));
             }
+            float cs, sn;
+            cis_cyc(fma((double)(D * j - pl) - kc, (double)sl[rc + p], (double)c0[rc + p]), cs, sn);
+            y[(rc + p) * Ko + j] = make_float2(fmaf(ar, cs, -(ai * sn)), fmaf(ar, sn, ai * cs));
         }
-        __syncthreads();
     }
 }
 
@@ -665,31 +646,29 @@ __global__ __launch_bounds__(256) void k_fir_p_global(const float2* __restrict__
 
 // ------------------------------------------------------------------------------------------------ device phases
 
-// c0, sl [Np, Cn, P]: for parent n at refs[n], child offset d[c], pulse p on the level's track (u, r0):
-// ddr = |ref + d - a| - |ref - a| without cancellation, c0 = wrap(ddr k0c), sl = ddr k1
+// c0, sl [Np, Cn, P]: the range change ddr = |q + d| - |q| from parent n's centre to child c's (offset d[c]) seen
+// from pulse p of the level's track a = r0[p] u[p], with q = refs[n] - a, written without the difference of two
+// large ranges as ddr = (|d|^2 + 2 d.q) / (|q + d| + |q|); c0 = ddr k0c wrapped to [-1/2, 1/2], sl = ddr k1.
+// grid (ceil(P / 256), Np), block 256: a thread forms q and |q| for its (parent, pulse) once and runs over the
+// children, each write coalesced along pulses.
 __global__ __launch_bounds__(256) void k_device_phases(const double* __restrict__ refs, const double* __restrict__ u,
                                                         const double* __restrict__ r0, const double* __restrict__ d, int Np,
                                                         int Cn, int P, double k0c, double k1, float* __restrict__ c0,
                                                         float* __restrict__ sl)
 {
-    const size_t total = (size_t)Np * Cn * P;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-        const int p = (int)(i % P);
-        const int c = (int)((i / P) % Cn);
-        const int n = (int)(i / ((size_t)P * Cn));
-        const double rx = refs[3 * n], ry = refs[3 * n + 1], rz = refs[3 * n + 2];
-        const double ux = u[3 * p], uy = u[3 * p + 1], uz = u[3 * p + 2];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x, n = blockIdx.y;
+    if (p >= P || n >= Np) return;
+    const double rp = r0[p];
+    const double qx = refs[3 * n] - rp * u[3 * p], qy = refs[3 * n + 1] - rp * u[3 * p + 1], qz = refs[3 * n + 2] - rp * u[3 * p + 2];
+    const double near = sqrt(qx * qx + qy * qy + qz * qz);
+    for (int c = 0; c < Cn; ++c) {
         const double dx = d[3 * c], dy = d[3 * c + 1], dz = d[3 * c + 2];
-        const double r = r0[p];
-        const double uc = rx * ux + ry * uy + rz * uz;
-        const double rr = rx * rx + ry * ry + rz * rz;
-        const double wn = r * sqrt(1.0 + (rr - 2.0 * r * uc) / (r * r));
-        const double ud = dx * ux + dy * uy + dz * uz;
-        const double wd = r * ud - (dx * rx + dy * ry + dz * rz);
-        const double num = (dx * dx + dy * dy + dz * dz) - 2.0 * wd;
-        const double ddr = num / (sqrt(wn * wn + num) + wn);
-        c0[i] = (float)wrap_cyc(ddr * k0c);
-        sl[i] = (float)(ddr * k1);
+        const double fx = qx + dx, fy = qy + dy, fz = qz + dz;
+        const double far = sqrt(fx * fx + fy * fy + fz * fz);
+        const double ddr = (dx * dx + dy * dy + dz * dz + 2.0 * (dx * qx + dy * qy + dz * qz)) / (far + near);
+        const size_t o = ((size_t)n * Cn + c) * P + p;
+        c0[o] = (float)wrap_cyc(ddr * k0c);
+        sl[o] = (float)(ddr * k1);
     }
 }
 
@@ -1121,7 +1100,7 @@ int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, con
     const int PB = 8;
     const int budget = std::min(lds_limit(), 65536);
     int JB = 64;
-    auto need = [&](int jb) { const int W = D * (jb - 1) + L; return 2 * PB * W * 4 + 2 * PB * (W | 1) * 4 + L * 4; };
+    auto need = [&](int jb) { const int W = D * (jb - 1) + L; return PB * (W + L) * 8; };
     while (JB > 4 && need(JB) > budget) JB /= 2;
     if (need(JB) > budget || want_long) {
         // a window longer than any LDS tile: the direct form (FSH_ROT_KERNEL=long forces it, for tests)
@@ -1262,10 +1241,12 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
 int launch_device_phases(const double* refs, const double* u, const double* r0, const double* d, int Np, int Cn, int P,
                          double k0c, double k1, float* c0, float* sl, hipStream_t s)
 {
-    const size_t n = (size_t)Np * Cn * P;
-    const int blocks = (int)std::min<size_t>((n + 255) / 256, 1 << 16);
-    hipLaunchKernelGGL(k_device_phases, dim3(blocks), dim3(256), 0, s, refs, u, r0, d, Np, Cn, P, k0c, k1, c0, sl);
-    FSH_TRY(hipGetLastError());
+    for (int n0 = 0; n0 < Np; n0 += 65535) {
+        const int nn = std::min(65535, Np - n0);
+        hipLaunchKernelGGL(k_device_phases, dim3((P + 255) / 256, nn), dim3(256), 0, s, refs + 3 * (size_t)n0, u, r0, d, nn, Cn,
+                           P, k0c, k1, c0 + (size_t)n0 * Cn * P, sl + (size_t)n0 * Cn * P);
+        FSH_TRY(hipGetLastError());
+    }
     return OK;
 }
 

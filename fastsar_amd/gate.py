@@ -21,38 +21,49 @@ from .plan import C
 
 
 def taper(f, band, guard):
-    """1 over band, a raised cosine to 0 across the guard on either side (fastsar/patches.py:156-159)."""
-    t = np.clip(np.maximum(band[0] - f, f - band[1]) / guard, 0, 1)
-    return 0.5 * (1 + np.cos(np.pi * t))
+    """The gated spectrum's weight at frequencies f: 1 on the band [band[0], band[1]], falling to 0 over `guard` on
+    either side along half a period of a cosine, 0 beyond (the mosaic's taper, fastsar/patches.py:156-159)."""
+    beyond = np.maximum(np.maximum(band[0] - f, f - band[1]), 0.0) / guard      # guards crossed, 0 on the band
+    return 0.5 * (1 + np.cos(np.pi * np.minimum(beyond, 1.0)))
+
+
+def range_reach(ant, ref, nx, ny, spx, spy, e1, e2, pulses=257, lattice=5):
+    """The largest |range - reference range| from the image's points to the antenna: the image sampled on a
+    lattice x lattice grid of points spanning it, the track at `pulses` positions spread evenly over it."""
+    sel = np.unique(np.linspace(0, len(ant) - 1, min(len(ant), pulses)).astype(np.int64))
+    track = ant[sel]
+    r_ref = np.linalg.norm(track, axis=1) if ref is None else np.asarray(ref, np.float64)[sel]
+    frac = np.linspace(-0.5, 0.5, lattice)
+    pts = ((frac * nx * spx)[:, None, None] * e1 + (frac * ny * spy)[None, :, None] * e2).reshape(-1, 3)
+    return np.abs(np.linalg.norm(pts[None] - track[:, None], axis=2) - r_ref[:, None]).max()
 
 
 def gate_plan(ant, ref, fmin, df, K, nx, ny, spx, spy, e1, e2):
     """The gate of an nx x ny image about the origin. -> dict(npl: samples padded below the band, Kp: padded
     length, N: range bins kept, k0, k1: the gated spectrum's samples kept, taper [k1 - k0] float32 (with the scale
     Kp / N, which keeps the image's amplitude: the gated history sums N / Kp times fewer samples), fmin, df: of the
-    gated history). The reach of the image is measured at its 5 x 5 lattice of points from 257 pulses spread over
-    the aperture. (fastsar/api.py:272-300)"""
+    gated history). Decisions as fastsar/api.py:272-300 makes them.
+
+    The band's guard is an eighth of its width on either side, npl = ceil(guard / df) samples, padded to Kp, a fast
+    transform length. A range bin is c / (2 Kp df); the kept bins cover the image's range reach (range_reach) plus six
+    resolution cells of the guard band, on both sides of zero delay: N, even and of a fast length. Transformed back,
+    the N frequencies are spaced Kp df / N from the padded band's first, and those the taper does not zero are kept."""
     from scipy.fft import next_fast_len
     ant = np.asarray(ant, np.float64)
     e1, e2 = np.asarray(e1, np.float64), np.asarray(e2, np.float64)
-    band = (fmin, fmin + (K - 1) * df)
-    guard = 0.125 * (band[1] - band[0])
+    lo, hi = fmin, fmin + (K - 1) * df
+    guard = 0.125 * (hi - lo)
     npl = int(np.ceil(guard / df))
     Kp = next_fast_len(K + 2 * npl)
-    idx = np.unique(np.linspace(0, len(ant) - 1, min(len(ant), 257)).astype(np.int64))
-    a = ant[idx]
-    r0 = np.linalg.norm(a, axis=1) if ref is None else np.asarray(ref, np.float64)[idx]
-    t = np.linspace(-0.5, 0.5, 5)
-    pts = ((t * nx * spx)[:, None, None] * e1 + (t * ny * spy)[None, :, None] * e2).reshape(-1, 3)
-    reach = np.abs(np.linalg.norm(pts[None] - a[:, None], axis=2) - r0[:, None]).max() + 6 * C / (2 * guard)
+    reach = range_reach(ant, ref, nx, ny, spx, spy, e1, e2) + 6 * C / (2 * guard)
     N = next_fast_len(2 * int(np.ceil(reach * 2 * Kp * df / C)))
-    N += N % 2
-    df2, f0 = Kp * df / N, fmin - npl * df
-    T = taper(f0 + df2 * np.arange(N), band, guard)
-    keep = np.nonzero(T > 0)[0]
-    k0, k1 = int(keep[0]), int(keep[-1]) + 1
-    return dict(npl=npl, Kp=int(Kp), N=int(N), k0=k0, k1=k1, taper=(T[k0:k1] * (Kp / N)).astype(np.float32),
-                fmin=float(f0 + k0 * df2), df=float(df2))
+    N += N & 1
+    spacing, first = Kp * df / N, fmin - npl * df
+    weight = taper(first + spacing * np.arange(N), (lo, hi), guard)
+    live = np.flatnonzero(weight > 0)
+    k0, k1 = int(live[0]), int(live[-1]) + 1
+    return dict(npl=npl, Kp=int(Kp), N=int(N), k0=k0, k1=k1, taper=(weight[k0:k1] * (Kp / N)).astype(np.float32),
+                fmin=float(first + k0 * spacing), df=float(spacing))
 
 
 class GateDesc(ctypes.Structure):

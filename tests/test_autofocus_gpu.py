@@ -1,78 +1,106 @@
-"""Autofocus of a phase history through the hip former (fastsar_amd.autofocus.autofocus; fastsar/autofocus.py:124-165),
-on the reference's test scene (tests/test_autofocus.py:14-48 of the reference): 40 points over 3000 clutter
-scatterers, 190 pulses x 190 samples, 128 x 128 pixels of 0.5 m, a former with T = 16 and 2 levels. A known phase
-error per pulse (quadratic, higher-order polynomial, low-pass random walk) is put on the history; after autofocus
-the image must match the image of the error-free history within -25 dB and the estimated phase must match the
-error within 0.1 rad rms, less constant and linear terms (the reference's bars)."""
-from types import SimpleNamespace
+"""Autofocus (fastsar_amd.autofocus.autofocus; the reference's fastsar/autofocus.py:124-165) through the hip former.
 
+Scene and bars are the reference's fixture (tests/test_autofocus.py:14-48 there): seed 3, 0.6 m resolution over
+0.5 m pixels, 40 points (amplitude 2-10, random phase) within +-26 m over 3000 clutter scatterers (0.15 complex
+Gaussian) within +-30 m, 190 pulses x 190 samples at 5 km, 128 x 128 pixels, a former with T = 16 and 2 levels.
+The phase errors put on the pulses are its three: 8 t^2, 6 t^3 + 5 t^4 - 4 t^5 (t from -1 to 1 over the aperture) and
+a random walk smoothed by a 41-point Hann window and scaled to a 3 rad peak, each without its constant and linear
+parts. Bars: the autofocused image within -25 dB of the error-free one and the phase estimate within 0.1 rad rms of
+the error (less a line); an error-free history comes back within -25 dB with a flat estimate; PGA leaves an image of
+the points alone within -30 dB."""
 import numpy as np
 import pytest
 
-from fastsar_amd import sim, autofocus as af
+from fastsar_amd import ImageFormer, autofocus as af, sim
+from fastsar_amd.metrics import rel_db
 
 pytestmark = pytest.mark.gpu
 
 GRID = dict(nx=128, ny=128, spx=0.5, spy=0.5, e1=(0.0, 1.0, 0.0), e2=(1.0, 0.0, 0.0))
-FK = dict(backend='hip', T=16, levels=2)
-N_PT, N_CL = 40, 3000
+FORMER = dict(backend='hip', T=16, levels=2)
+POINTS, CLUTTER = 40, 3000
 
 
-def rel_db(a, b):
-    g = np.vdot(a, b) / np.vdot(a, a)
-    return 10 * np.log10(np.sum(np.abs(g * a - b) ** 2) / np.sum(np.abs(b) ** 2))
+def without_line(phase):
+    """phase less its least-squares line over the pulse index (what PGA cannot see)"""
+    k = np.arange(len(phase), dtype=np.float64)
+    return phase - np.polyval(np.polyfit(k, phase, 1), k)
+
+
+class Scene:
+    """The fixture's collection, scatterers, history and phase errors (draws in the fixture's order)."""
+
+    def __init__(self):
+        rng = np.random.default_rng(3)
+        self.col = sim.make_collect(res=0.6, scene=64.0, r0=5e3)
+        xy_pt = [rng.uniform(-26, 26, POINTS), rng.uniform(-26, 26, POINTS)]
+        xy_cl = [rng.uniform(-30, 30, CLUTTER), rng.uniform(-30, 30, CLUTTER)]
+        self.pos = np.concatenate([np.column_stack(xy_pt + [np.zeros(POINTS)]), np.column_stack(xy_cl + [np.zeros(CLUTTER)])])
+        a_pt = rng.uniform(2, 10, POINTS) * np.exp(2j * np.pi * rng.random(POINTS))
+        a_cl = 0.15 * (rng.standard_normal(CLUTTER) + 1j * rng.standard_normal(CLUTTER))
+        self.amp = np.concatenate([a_pt, a_cl])
+        self.S = sim.simulate_brute(self.col, self.pos, self.amp).astype(np.complex64)
+        P = self.col.Np
+        t = np.linspace(-1.0, 1.0, P)
+        hann = np.hanning(41)
+        steps = np.cumsum(rng.standard_normal(P + 40))
+        walk = without_line(np.convolve(steps, hann / hann.sum(), 'valid')[:P])
+        self.errors = {'quadratic': without_line(8 * t ** 2),
+                       'polynomial': without_line(6 * t ** 3 + 5 * t ** 4 - 4 * t ** 5),
+                       'random walk': walk * (3.0 / np.abs(walk).max())}
+        c = self.col
+        self.former = ImageFormer(c.ant, c.fmin, c.df, c.K, **GRID, **FORMER)
+        self.focused = self.former(self.S)
+
+    def autofocus(self, S):
+        c = self.col
+        return af.autofocus(S, c.ant, c.fmin, c.df, **GRID, **FORMER)
 
 
 @pytest.fixture(scope='module')
 def scene():
-    from fastsar_amd import ImageFormer
-    rng = np.random.default_rng(3)
-    col = sim.make_collect(res=0.6, scene=64.0, r0=5e3)
-    pos = np.concatenate([np.stack([rng.uniform(-26, 26, N_PT), rng.uniform(-26, 26, N_PT), np.zeros(N_PT)], 1),
-                          np.stack([rng.uniform(-30, 30, N_CL), rng.uniform(-30, 30, N_CL), np.zeros(N_CL)], 1)])
-    amp = np.concatenate([rng.uniform(2, 10, N_PT) * np.exp(2j * np.pi * rng.random(N_PT)),
-                          0.15 * (rng.standard_normal(N_CL) + 1j * rng.standard_normal(N_CL))])
-    S = sim.simulate_brute(col, pos, amp).astype(np.complex64)
-    P = col.Np
-    p = np.arange(P, dtype=np.float64)
-    t = 2 * p / (P - 1) - 1
-    walk = np.convolve(np.cumsum(rng.standard_normal(P + 40)), np.hanning(41) / np.hanning(41).sum(), 'valid')[:P]
-    errs = {'quadratic': 8 * t ** 2, 'polynomial': 6 * t ** 3 + 5 * t ** 4 - 4 * t ** 5, 'random walk': walk}
-    former = ImageFormer(col.ant, col.fmin, col.df, col.K, **GRID, **FK)
-    print(f'autofocus scene: {P} pulses, {col.K} samples, {N_PT} points over {N_CL} clutter', flush=True)
-    return SimpleNamespace(col=col, pos=pos, amp=amp, S=S, p=p, errs=errs, former=former, ref=former(S))
+    s = Scene()
+    print(f'autofocus scene: {s.col.Np} pulses x {s.col.K} samples, {POINTS} points over {CLUTTER} clutter', flush=True)
+    return s
 
 
-@pytest.mark.parametrize('name', ['quadratic', 'polynomial', 'random walk'])
-def test_phase_error_removed(scene, name):
-    c, S, former, ref = scene.col, scene.S, scene.former, scene.ref
-    e = af._remove_line(np.array(scene.errs[name]), scene.p)
-    if name == 'random walk':
-        e *= 3 / np.abs(e).max()
-    Se = S * np.exp(1j * e)[:, None].astype(np.complex64)
-    img, phi = af.autofocus(Se, c.ant, c.fmin, c.df, **GRID, **FK)
-    res = af._remove_line(phi - e, scene.p)
-    before, after = rel_db(former(Se), ref), rel_db(img, ref)
-    print(f'{name}: peak {np.abs(e).max():.1f} rad: image {before:.1f} dB -> {after:.1f} dB, residual {res.std():.3f} rad')
-    assert before > -10
-    assert res.std() < 0.1 and after < -25, (name, res.std(), after)
+@pytest.mark.parametrize('kind', ['quadratic', 'polynomial', 'random walk'])
+def test_known_phase_error_is_removed(scene, kind):
+    err = scene.errors[kind]
+    blurred = (scene.S * np.exp(1j * err)[:, None]).astype(np.complex64)
+    img, estimate = scene.autofocus(blurred)
+    residual = np.sqrt(np.mean(without_line(estimate - err) ** 2))
+    db_before = rel_db(scene.former(blurred).ravel(), scene.focused.ravel())
+    db_after = rel_db(img.ravel(), scene.focused.ravel())
+    print(f'{kind}: {np.abs(err).max():.1f} rad peak; image {db_before:.1f} -> {db_after:.1f} dB; '
+          f'residual {residual:.3f} rad rms')
+    assert db_before > -10, 'the error must blur the image for the test to mean anything'
+    assert db_after < -25
+    assert residual < 0.1
 
 
-def test_focused_image_unchanged(scene):
-    c, S, former, ref = scene.col, scene.S, scene.former, scene.ref
-    ramp = np.exp(-1j * af.deramp_phase(c.ant, c.fmin, c.df, c.K, **GRID))
-    img, _, hist = af.pga(ref * ramp)
-    img2, phi = af.autofocus(S, c.ant, c.fmin, c.df, **GRID, **FK)
-    assert rel_db(img, ref * ramp) < -25 and rel_db(img2, ref) < -25
-    assert af._remove_line(phi, scene.p).std() < 0.1
-    pts = former(sim.simulate_brute(c, scene.pos[:N_PT], scene.amp[:N_PT]).astype(np.complex64)) * ramp
-    img, _, _ = af.pga(pts)
-    assert rel_db(img, pts) < -30
+def test_error_free_history_comes_back(scene):
+    img, estimate = scene.autofocus(scene.S)
+    assert rel_db(img.ravel(), scene.focused.ravel()) < -25
+    assert np.sqrt(np.mean(without_line(estimate) ** 2)) < 0.1
 
 
-def test_autofocus_arguments(scene):
+def test_pga_keeps_a_focused_image(scene):
+    """PGA itself (the polar-format deramped image in, the corrected image out) on the image of the points alone"""
+    c = scene.col
+    deramp = np.exp(-1j * af.deramp_phase(c.ant, c.fmin, c.df, c.K, **GRID))
+    pts = sim.simulate_brute(c, scene.pos[:POINTS], scene.amp[:POINTS]).astype(np.complex64)
+    img = scene.former(pts) * deramp
+    out, _, _ = af.pga(img)
+    assert rel_db(out.ravel(), img.ravel()) < -30
+    full = scene.focused * deramp
+    out, _, _ = af.pga(full)
+    assert rel_db(out.ravel(), full.ravel()) < -25
+
+
+def test_autofocus_refuses_bad_arguments(scene):
     c = scene.col
     with pytest.raises(ValueError):
         af.autofocus(scene.S, c.ant, c.fmin, c.df, **GRID, algorithm='exact', backend='hip')
     with pytest.raises(ValueError):
-        af.autofocus(scene.S[0], c.ant, c.fmin, c.df, **GRID, **FK)
+        af.autofocus(scene.S[0], c.ant, c.fmin, c.df, **GRID, **FORMER)

@@ -61,6 +61,7 @@ def lib():
         'fsh_mem_info': ([ctypes.POINTER(c_size_t), ctypes.POINTER(c_size_t)], c_int),
         'fsh_ffbp_level_forms': ([vp, P_i32, c_int], c_int),
         'fsh_ffbp_generation': ([vp, ctypes.POINTER(ctypes.c_ulonglong)], c_int),
+        'fsh_ffbp_adopt': ([vp, vp, P_i32], c_int),
         'fsh_rot_fir_k': ([vp, c_int, c_int, c_int, vp, vp, c_int, vp, c_int, c_int, c_int, c_int, c_double, vp, vp], c_int),
         'fsh_fir_p': ([vp, c_int, c_int, c_int, vp, c_int, c_int, c_int, c_int, vp, vp], c_int),
         'fsh_level_fused': ([vp, c_int, c_int, c_int, vp, vp, c_int, vp, c_int, c_int, c_int, c_int, vp, c_int, c_int, c_int,

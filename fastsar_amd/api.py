@@ -152,7 +152,7 @@ class ImageFormer:
     histories: former = ImageFormer(ant, fmin, df, K, nx, ny, spx, spy, e1, e2); img = former(S).
     (fastsar/api.py:312-677)
 
-    T: 'auto' (the largest of 32 and 16 whose predicted error meets target_db) or 4, 8, 16, 32. levels: None
+    T: 'auto' (the largest of 32, 16, 8 and 4 whose predicted error meets target_db) or 4, 8, 16, 32. levels: None
     (three, more where three cannot split an axis) or a count. aperture_weight(points, pulses) -> [n, m]: per-pixel
     pulse weights applied in the final stage. ref [P]: each pulse's reference range when it is not |ant|.
     precision: 'float32', or 'float16' (the two-pass levels' intermediate rows stored in float16). range_gate: True
@@ -185,8 +185,10 @@ class ImageFormer:
         self.precision = precision
         self.predicted_error_db = None
         if T == 'auto':
+            # the largest of 32, 16, 8, 4 whose predicted error meets target_db: the hip final stage takes all four
+            # (the reference's cpu and cuda stages take 32 and 16, its jax program 8 too; fastsar/ffbp2.py:300-307)
             fmax = float(fmin) + self.K * float(df)
-            T, err = _plan.choose_T(self.ant, fmax, nx, ny, spx, spy, e1, e2, target_db)
+            T, err = _plan.choose_T(self.ant, fmax, nx, ny, spx, spy, e1, e2, target_db, choices=(32, 16, 8, 4))
             if err > target_db:
                 warnings.warn(f'predicted error {err:.1f} dB at T={T} misses target {target_db:.1f} dB '
                               '(short range for this pixel size)')
@@ -262,18 +264,28 @@ class ImageFormer:
         pulse p times exp(sign 2 pi i f_k coef[p]), f_k = fmin + k df (fastsar/io.py:384-420; fastsar/cphd.py:401 moves
         the reference to the grid centre with coef = 2 dref / c, sign -1). coef None removes it."""
         from . import _hip
+        if coef is not None:
+            coef = np.asarray(coef, np.float64)
+            if coef.shape != (self.P,) or not np.all(np.isfinite(coef)):
+                raise ValueError(f'coef must hold one finite value per pulse ({self.P}), got shape {coef.shape}')
+            if sign not in (-1, 1):
+                raise ValueError('sign must be +1 or -1')
         if self._gate is not None:
-            raise ValueError('a gated former takes its history re-referenced on the host')
+            # a gated former re-references the whole history before its gate cuts it, on the history's own grid
+            self._gate.set_ramp(None if coef is None else np.stack([sign * self._fmin * coef, sign * self._df * coef], 1))
+            return
         if coef is None:
             _hip.check(_hip.lib().fsh_ffbp_set_ramp(self._form.h, None), 'ffbp_set_ramp')
             return
-        coef = np.asarray(coef, np.float64)
-        if coef.shape != (self.P,) or not np.all(np.isfinite(coef)):
-            raise ValueError(f'coef must hold one finite value per pulse ({self.P}), got shape {coef.shape}')
-        if sign not in (-1, 1):
-            raise ValueError('sign must be +1 or -1')
         ab = np.ascontiguousarray(np.stack([sign * self._fmin * coef, sign * self._df * coef], 1))
         _hip.check(_hip.lib().fsh_ffbp_set_ramp(self._form.h, ab.ctypes.data_as(ctypes.c_void_p)), 'ffbp_set_ramp')
+
+    def adopt_buffers(self, donor):
+        """Take over the device buffers of another ImageFormer whose plan has this one's shape (the formers of a
+        mosaic's patches of equal shape): -> True when taken. The donor makes its own again if called later."""
+        if not isinstance(donor, ImageFormer) or donor is self:
+            return False
+        return self._form.adopt(donor._form)
 
     def stage(self, S):
         """The hip former takes the history as it is: former(former.stage(S)) equals former(S). (fastsar/api.py:471-480)"""

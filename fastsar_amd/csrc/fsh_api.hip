@@ -739,6 +739,38 @@ int fsh_ffbp_info(void* handle, int* ng_used, size_t* device_bytes)
     return OK;
 }
 
+int fsh_ffbp_adopt(void* handle, void* donor, int* adopted)
+{
+    if (!handle || !donor || handle == donor || !adopted) { set_error("ffbp_adopt: invalid argument"); return E_INVALID; }
+    *adopted = 0;
+    Former* f = static_cast<Former*>(handle);
+    Former* d = static_cast<Former*>(donor);
+    if (d->ng < 1) return OK;                              // the donor has formed nothing yet: nothing to take
+    size_t mb, mp, tiles;
+    f->group_sizes(d->ng, mb, mp, tiles);
+    if (mb != d->bufsz || mp != d->phsz || tiles != d->geosz || f->p.Pf != d->p.Pf) return OK;   // another shape
+    FSH_TRY(hipStreamSynchronize(d->stream));
+    FSH_TRY(hipStreamSynchronize(f->stream));
+    f->release_group();
+    f->buf[0] = d->buf[0]; f->buf[1] = d->buf[1]; f->c0b = d->c0b; f->slb = d->slb;
+    f->ux = d->ux; f->uy = d->uy; f->tgeo = d->tgeo;
+    f->bufsz = d->bufsz; f->phsz = d->phsz; f->geosz = d->geosz;
+    f->ng = d->ng;
+    f->auto_cap = d->auto_cap;
+    ++f->gen;
+    d->buf[0] = d->buf[1] = nullptr; d->c0b = d->slb = nullptr; d->ux = d->uy = d->tgeo = nullptr;
+    d->bufsz = d->phsz = d->geosz = 0;
+    d->ng = 0;
+    ++d->gen;
+    if (!f->hist && d->hist && f->p.P == d->p.P && f->p.K == d->p.K) {
+        f->hist = d->hist;
+        d->hist = nullptr;
+    }
+    if (!f->stage.ready() && d->stage.ready()) std::swap(f->stage, d->stage);
+    *adopted = 1;
+    return OK;
+}
+
 int fsh_ffbp_set_profile(void* handle, int on)
 {
     if (!handle) { set_error("ffbp_set_profile: null handle"); return E_INVALID; }

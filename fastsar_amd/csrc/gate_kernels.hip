@@ -30,10 +30,12 @@ int fft_st(hipfftResult r, const char* what)
     return r == HIPFFT_ALLOC_FAILED ? E_NOMEM : E_RUNTIME;
 }
 
-// z [n, Kp]: z[p, npl + k] = S[p0 + p, k] wp wk (zero elsewhere); the first non-finite pulse into bad
+// z [n, Kp]: z[p, npl + k] = S[p0 + p, k] (times the re-reference phase, then wp wk) (zero elsewhere); the first
+// non-finite pulse into bad
 __global__ __launch_bounds__(256) void k_gate_pad(const float2* __restrict__ S, const float* __restrict__ wp,
                                                   const float* __restrict__ wk, int n, int K, int npl, int Kp, int p0,
-                                                  float2* __restrict__ z, int* __restrict__ bad)
+                                                  float2* __restrict__ z, int* __restrict__ bad,
+                                                  const double2* __restrict__ ramp)
 {
     const size_t total = (size_t)n * Kp;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
@@ -42,6 +44,7 @@ __global__ __launch_bounds__(256) void k_gate_pad(const float2* __restrict__ S, 
         if (k >= 0 && k < K) {
             v = S[(size_t)p * K + k];
             if (!(isfinite(v.x) && isfinite(v.y))) atomicMin(bad, p0 + p);
+            v = apply_ramp(v, ramp, p0 + p, k);
             if (wp) {
                 const float g = wp[p0 + p] * wk[k];
                 v.x *= g;
@@ -87,6 +90,8 @@ struct Gate {
     float2* z = nullptr;           // [chunk, Kp], transformed in place
     float2* v = nullptr;           // [chunk, N], transformed in place
     int* bad = nullptr;
+    double* ramp = nullptr;        // [P] (a, b) cycles, fsh_gate_set_ramp
+    bool use_ramp = false;
     int chunk = 0;
     hipfftHandle inv = 0, fwd = 0;
     bool have_inv = false, have_fwd = false;
@@ -95,7 +100,7 @@ struct Gate {
     {
         if (have_inv) hipfftDestroy(inv);
         if (have_fwd) hipfftDestroy(fwd);
-        dev_free(taper); dev_free(wp); dev_free(wk); dev_free(hist); dev_free(z); dev_free(v); dev_free(bad);
+        dev_free(taper); dev_free(wp); dev_free(wk); dev_free(hist); dev_free(z); dev_free(v); dev_free(bad); dev_free(ramp);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -190,7 +195,8 @@ int fsh_gate_run(void* handle, const void* S, int S_on_device, float* out, int* 
             src = g->hist;
         }
         const int blocks = 4096;
-        hipLaunchKernelGGL(k_gate_pad, dim3(blocks), dim3(256), 0, g->stream, src, g->wp, g->wk, n, d.K, d.npl, d.Kp, p0, g->z, g->bad);
+        hipLaunchKernelGGL(k_gate_pad, dim3(blocks), dim3(256), 0, g->stream, src, g->wp, g->wk, n, d.K, d.npl, d.Kp, p0, g->z, g->bad,
+                           g->use_ramp ? reinterpret_cast<const double2*>(g->ramp) : nullptr);
         FSH_TRY(hipGetLastError());
         if ((st = fft_st(hipfftExecC2C(g->inv, (hipfftComplex*)g->z, (hipfftComplex*)g->z, HIPFFT_BACKWARD), "hipfftExecC2C")))
             return st;
@@ -210,6 +216,22 @@ int fsh_gate_run(void* handle, const void* S, int S_on_device, float* out, int* 
         set_error("phase history has non-finite samples (NaN or inf), the first in pulse " + std::to_string(badh));
         return E_NONFINITE;
     }
+    return OK;
+}
+
+int fsh_gate_set_ramp(void* handle, const double* ab)
+{
+    if (!handle) { set_error("gate_set_ramp: null handle"); return E_INVALID; }
+    Gate* g = static_cast<Gate*>(handle);
+    if (!ab) {
+        g->use_ramp = false;
+        return OK;
+    }
+    int st;
+    if (!g->ramp && (st = dev_alloc((void**)&g->ramp, (size_t)g->d.P * 2 * sizeof(double)))) return st;
+    FSH_TRY(hipMemcpyAsync(g->ramp, ab, (size_t)g->d.P * 2 * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    FSH_TRY(hipStreamSynchronize(g->stream));
+    g->use_ramp = true;
     return OK;
 }
 

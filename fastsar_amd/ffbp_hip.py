@@ -119,6 +119,14 @@ class HipFFBP:
         _hip.check(_hip.lib().fsh_ffbp_info(self.h, ctypes.byref(ng), ctypes.byref(nb)), 'ffbp_info')
         return dict(group=ng.value, device_bytes=nb.value)
 
+    def adopt(self, donor):
+        """Take over the group buffers of another HipFFBP whose plan has this one's shape (fsh_ffbp_adopt): -> True
+        when taken (the donor must make its buffers again to form), False when the shapes differ or the donor has
+        formed nothing yet."""
+        got = ctypes.c_int(0)
+        _hip.check(_hip.lib().fsh_ffbp_adopt(self.h, donor.h, ctypes.byref(got)), 'ffbp_adopt')
+        return bool(got.value)
+
     def generation(self):
         """A counter that changes whenever the former makes its group buffers again."""
         g = ctypes.c_ulonglong()

@@ -78,7 +78,8 @@ def _bind():
         L.fsh_gate_create.argtypes = [ctypes.POINTER(GateDesc), ctypes.POINTER(vp)]
         L.fsh_gate_run.argtypes = [vp, vp, ci, vp, _hip.P_i32]
         L.fsh_gate_destroy.argtypes = [vp]
-        for n in ('fsh_gate_create', 'fsh_gate_run', 'fsh_gate_destroy'):
+        L.fsh_gate_set_ramp.argtypes = [vp, vp]
+        for n in ('fsh_gate_create', 'fsh_gate_run', 'fsh_gate_destroy', 'fsh_gate_set_ramp'):
             getattr(L, n).restype = ci
         L._gate_bound = True
     return L
@@ -109,6 +110,18 @@ class RangeGate:
         h = ctypes.c_void_p()
         _hip.check(L.fsh_gate_create(ctypes.byref(d), ctypes.byref(h)), 'range gate')
         self._h = h.value
+
+    def set_ramp(self, ab):
+        """Per pulse (a_p, b_p) in cycles [P, 2] (or None): sample k of pulse p times exp(2 pi i (a_p + b_p k)) before
+        the window and the gate (a re-reference of the pulses' ranges)."""
+        L = _bind()
+        if ab is None:
+            _hip.check(L.fsh_gate_set_ramp(ctypes.c_void_p(self._h), None), 'range gate ramp')
+            return
+        ab = np.ascontiguousarray(ab, np.float64)
+        if ab.shape != (self.P, 2) or not np.all(np.isfinite(ab)):
+            raise ValueError(f'the ramp must be [{self.P}, 2] finite cycles, got shape {ab.shape}')
+        _hip.check(L.fsh_gate_set_ramp(ctypes.c_void_p(self._h), ab.ctypes.data_as(ctypes.c_void_p)), 'range gate ramp')
 
     def __call__(self, S, out=None):
         L = _bind()

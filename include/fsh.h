@@ -202,6 +202,12 @@ int fsh_ffbp_stage_ms(void* handle, float* ms, int n);
 /* Profiling (no reference counterpart; the analog of ffbp_cuda.PROFILE, fastsar/ffbp_cuda.py:635-652): when on,
  * every launch is bracketed by events on the former's stream and waited for; fsh_ffbp_profile then gives, per
  * launch of the last call, its stage (the order above), its level (nlev for the final stage) and its device ms. */
+/* Take over the group buffers (and the host-history upload buffer and pinned staging ring) of another former whose
+ * plan has the same shape, so that a former made for each patch of a mosaic allocates none of them again;
+ * *adopted = 1 when taken (the donor keeps its constants and must make its buffers again to form), 0 when the shapes
+ * differ or the donor has no buffers yet. Both formers' streams are waited for. The analog of the reference's
+ * compiled-program cache per plan signature (fastsar/api.py:661-677, fastsar/ffbp2.py:259-273). */
+int fsh_ffbp_adopt(void* handle, void* donor, int* adopted);
 int fsh_ffbp_set_profile(void* handle, int on);
 int fsh_ffbp_profile(void* handle, int* stage, int* level, float* ms, int cap, int* count);
 int fsh_ffbp_destroy(void* handle);
@@ -267,6 +273,10 @@ int fsh_gate_create(const fsh_gate_desc* desc, void** handle);
 /* S [P, K] complex64 (host or device) -> out [P, k1 - k0] complex64 (device); FSH_ENONFINITE with *bad_pulse, the
  * first pulse holding a NaN or inf. */
 int fsh_gate_run(void* handle, const void* S, int S_on_device, float* out, int* bad_pulse);
+/* The phase re-reference of each pulse applied before the window, as fsh_ffbp_set_ramp: ab [P, 2] float64 (host) in
+ * cycles, sample k of pulse p times exp(2 pi i (a_p + b_p k)); NULL removes it. Moves a patch's history to its own
+ * reference ranges before the gate cuts it (fastsar/patches.py:378-500, the re-reference of patch_history). */
+int fsh_gate_set_ramp(void* handle, const double* ab);
 int fsh_gate_destroy(void* handle);
 
 /* Exact backprojection onto points in float64 (device pointers): S [P, K] complex64 windowed by wp, wk (or NULL),

@@ -130,10 +130,18 @@ def cpu_baseline(col, grid, S_host, images, tiles, side, ref_vals, T_cpu=16):
                 error_tiles=f'{len(tiles)} x {side}^2', cpu_model=cpu_model(), threads=threads)
 
 
-def exact_leg(col, grid, dS, tiles, side, ref_vals, steps):
+def exact_leg(col, grid, dS, tiles, side, ref_vals, steps, full=True):
     """Exact backprojection (ExactFormer) of the same history: the whole grid with cubic reads (time of one warm call
-    and its error on the accuracy tiles against the float64 checker), and a 4,096 x 4,096 grid about the scene centre
-    with cubic and with linear reads (median of `steps` calls). Pixel-pulse pairs per second."""
+    and its error on the accuracy tiles against the float64 checker; skipped with full=False), and a 4,096 x 4,096 grid
+    about the scene centre with cubic and with linear reads (median of `steps` calls). Pixel-pulse pairs per second."""
+    out = {}
+    if full:
+        out.update(_exact_full(col, grid, dS, tiles, side, ref_vals))
+    _exact_4096(col, grid, dS, steps, out)
+    return out
+
+
+def _exact_full(col, grid, dS, tiles, side, ref_vals):
     from fastsar_amd import _hip
     from fastsar_amd.exact import ExactFormer
     from fastsar_amd.metrics import rel_db
@@ -152,7 +160,12 @@ def exact_leg(col, grid, dS, tiles, side, ref_vals, steps):
     out['full'] = dict(interp='cubic', tile=list(f.tile), ms=round(t * 1e3, 1), gpairs_per_s=round(pairs / t / 1e9, 1),
                        error_db=round(err, 2), error_tiles=f'{len(tiles)} x {side}^2')
     log(f'exact backprojection {grid["nx"]}x{grid["ny"]} from {col.Np} x {col.K}, cubic: {out["full"]}')
-    del img, h, f
+    return out
+
+
+def _exact_4096(col, grid, dS, steps, out):
+    from fastsar_amd import _hip
+    from fastsar_amd.exact import ExactFormer
     g4 = dict(grid, nx=4096, ny=4096)
     img = _hip.DeviceArray((4096, 4096), np.complex64)
     for interp in ('cubic', 'linear'):
@@ -170,7 +183,6 @@ def exact_leg(col, grid, dS, tiles, side, ref_vals, steps):
         out[f'4096x4096_{interp}'] = dict(ms=round(t * 1e3, 2), gpairs_per_s=round(pairs / t / 1e9, 1))
         log(f'exact backprojection 4096x4096 from {col.Np} x {col.K}, {interp}: {out[f"4096x4096_{interp}"]}')
         del f
-    return out
 
 
 def measured_traffic(label, former, args):
@@ -329,6 +341,8 @@ def main():
     ap.add_argument('--cpu-images', type=int, default=1, help='whole images the reference CPU former forms (median)')
     ap.add_argument('--host-steps', type=int, default=5, help='also time calls from a host history to a host image (median)')
     ap.add_argument('--no-exact', action='store_true', help='skip the exact-backprojection leg')
+    ap.add_argument('--exact', action='store_true', help='the exact-backprojection leg alone (for profiles)')
+    ap.add_argument('--no-full', action='store_true', help='with --exact: the 4096 x 4096 grids only')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-profile', action='store_true', help='timed steps only (for counter runs under rocprofv3)')
     ap.add_argument('--calibrate-copy', action='store_true', help='also run the plain copy kernel on 1 GiB three times')
@@ -373,6 +387,22 @@ def main():
     t0 = time.perf_counter()
     dS = simulate_device(col, pos, amp, clutter_sigma=sim.clutter_sigma(amp, cl_db), seed=seed)
     log(f'phase history {col.Np} x {col.K} generated on the device in {time.perf_counter() - t0:.2f} s')
+    if args.exact:
+        tiles, pts, side, ref_vals = [], None, 0, None
+        if not args.no_full:
+            tiles, pts, side = accuracy_points(grid, args.tiles, 256, seed=7)
+            nfft = 1 << int(np.ceil(np.log2(args.upsample * col.K)))
+            wp, wk = _window(col.Np, col.K)
+            ref_vals = bp_points_f64_device(dS, col.ant, np.linalg.norm(col.ant, axis=1), col.fmin, col.df, pts, nfft, True,
+                                            wp, wk, chunk=max(1, (1 << 30) // (nfft * 16)))
+        ex = exact_leg(col, grid, dS, tiles, side, ref_vals, args.steps, full=not args.no_full)
+        lin = ex['4096x4096_linear']
+        print(json.dumps(dict(metric='exact backprojection Gpixel·pulses/s, 4096x4096 from 16384 x 8192, linear reads',
+                              value=lin['gpairs_per_s'], unit='Gpixel·pulses/s', n_gpus=1, steps=args.steps, warmup=1,
+                              ms_per_step=lin['ms'], higher_is_better=True, scaling='weak', vs_baseline=None, dtype='f32',
+                              data='synthetic', config=dict(workload='exact backprojection on the 12288x8192 benchmark '
+                                                                     'geometry', exact_form=ex))), flush=True)
+        return
     hist, hist_host = dS, None
     if args.history != 'complex64':
         from fastsar_amd import precision as pr

@@ -202,7 +202,14 @@ def bench_batch(args):
     """The batched workload: 48 images of uneven size (256^2 to 4096^2 px, 512 to 8192 pulses x 1024 to 4096
     samples, spotlight and 2-degree-squint patches), each former's final tile and levels chosen by the autotuner,
     formed by one hipGraph launch; against the same images formed by one former call each."""
-    from fastsar_amd import _hip, sim
+    out = run_batch(args, tune=True)
+    print(json.dumps(out), flush=True)
+
+
+def run_batch(args, tune):
+    """The batch of bench_batch -> its result line (dict). tune=False: each image's final tile chosen by its predicted
+    error alone (T 8, else 4, four levels), no timing of candidates (the default run's bounded form)."""
+    from fastsar_amd import ImageFormer, _hip, plan as pl, sim
     from fastsar_amd.batch import Batch, autotune
     from fastsar_amd.exact import simulate_device, bp_points_f64_device
     from fastsar_amd.metrics import rel_db
@@ -213,13 +220,19 @@ def bench_batch(args):
     for i, sc in enumerate(scenes):
         col, g = sc['col'], sc['grid']
         dS = simulate_device(col, sc['pos'], sc['amp'], clutter_sigma=sc['sigma'], seed=100 + i)
-        f = autotune(col.ant, col.fmin, col.df, col.K, **g, S=dS, target_db=args.batch_target, atten=args.batch_atten,
-                     candidates=BATCH_CANDIDATES)
+        if tune:
+            f = autotune(col.ant, col.fmin, col.df, col.K, **g, S=dS, target_db=args.batch_target, atten=args.batch_atten,
+                         candidates=BATCH_CANDIDATES)
+        else:
+            T, _ = pl.choose_T(col.ant, col.fmin + col.K * col.df, g['nx'], g['ny'], g['spx'], g['spy'], g['e1'], g['e2'],
+                               args.batch_target, choices=(8, 4))
+            f = ImageFormer(col.ant, col.fmin, col.df, col.K, **g, T=T, levels=4, atten=args.batch_atten)
+            f.tuned = dict(T=T, levels=4, ms=None, tried=0)
         formers.append(f)
         hists.append(dS)
         log(f'image {i}: {g["nx"]}^2 from {col.Np} x {col.K}, squint {sc["squint"]:g}: T={f.T}, {f.levels} levels '
             f'({f.tuned["tried"]} tried, {f.tuned["ms"] and round(f.tuned["ms"], 2)} ms)')
-    log(f'histories and autotuned formers in {time.perf_counter() - t0:.1f} s')
+    log(f'histories and formers in {time.perf_counter() - t0:.1f} s')
     b = Batch(formers, hists, streams=args.batch_streams)
     for _ in range(args.warmup):
         b.run()
@@ -254,12 +267,14 @@ def bench_batch(args):
     choice = {}
     for f in formers:
         choice[f'T{f.T}/L{f.levels}'] = choice.get(f'T{f.T}/L{f.levels}', 0) + 1
+    del b, formers, hists
     out = dict(metric='aggregate Gpixel·pulses/s, 48 uneven images in one hipGraph launch', value=round(pairs / ms / 1e6, 2),
                unit='Gpixel·pulses/s', n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 2),
                higher_is_better=True, scaling='weak', vs_baseline=None, dtype='f32', data='synthetic',
                config=dict(workload=(f'{len(scenes)} images, sides 256-4096 px log-uniform (seed 5), 512-8192 pulses x '
-                                     '1024-4096 samples, spotlight and 2-degree-squint patches, autotuned T and levels, '
-                                     f'{args.batch_streams} streams in one graph'),
+                                     '1024-4096 samples, spotlight and 2-degree-squint patches, '
+                                     + ('autotuned T and levels, ' if tune else 'T by predicted error, 4 levels, ')
+                                     + f'{args.batch_streams} streams in one graph'),
                            pixel_pulses=pairs, streams=args.batch_streams,
                            latency_ms_batched=dict(p50=round(float(np.percentile(done, 50)), 2),
                                                    p99=round(float(np.percentile(done, 99)), 2)),
@@ -267,8 +282,8 @@ def bench_batch(args):
                                                   p99=round(float(np.percentile(each, 99)), 2)),
                            loop_ms=round(lms, 2), loop_value=round(pairs / lms / 1e6, 2), speedup_vs_loop=round(lms / ms, 3),
                            error_db_worst=round(float(errs.max()), 2), error_db_median=round(float(np.median(errs)), 2),
-                           autotune=choice, target_db=args.batch_target, atten_db=args.batch_atten))
-    print(json.dumps(out), flush=True)
+                           choice=choice, target_db=args.batch_target, atten_db=args.batch_atten))
+    return out
 
 
 def bench_learned(args):
@@ -341,6 +356,7 @@ def main():
     ap.add_argument('--cpu-images', type=int, default=1, help='whole images the reference CPU former forms (median)')
     ap.add_argument('--host-steps', type=int, default=5, help='also time calls from a host history to a host image (median)')
     ap.add_argument('--no-exact', action='store_true', help='skip the exact-backprojection leg')
+    ap.add_argument('--no-batch', action='store_true', help='skip the 48-image batch in the default run')
     ap.add_argument('--exact', action='store_true', help='the exact-backprojection leg alone (for profiles)')
     ap.add_argument('--no-full', action='store_true', help='with --exact: the 4096 x 4096 grids only')
     ap.add_argument('--no-cpu', action='store_true')
@@ -582,6 +598,17 @@ def main():
     if not args.no_exact and not args.small and args.history == 'complex64':
         exact = exact_leg(col, grid, dS, tiles, side, ref_vals, args.steps)
 
+    batch = None
+    if not args.no_batch and not args.small and args.history == 'complex64':
+        # the 48-image batch in its bounded form (no candidate timing); bench.py --batch autotunes it. The image's
+        # former and its buffers are released first.
+        fT, flev, fpred = former.T, former.levels, former.predicted_error_db
+        del himg, img
+        former = None
+        bl = run_batch(args, tune=False)
+        former = type('FormerFacts', (), dict(T=fT, levels=flev, predicted_error_db=fpred))
+        batch = dict(bl['config'], ms=bl['ms_per_step'], value=bl['value'], metric=bl['metric'])
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -605,7 +632,7 @@ def main():
                            predicted_error_db=round(former.predicted_error_db, 2), parallelism='single GPU',
                            host_to_host_ms=None if h2h is None else round(h2h * 1e3, 1),
                            host_to_host_steps=args.host_steps if h2h is not None else 0, learned_form=learned_form,
-                           exact_form=exact),
+                           exact_form=exact, batch_form=batch),
                roofline=roofline, cpu_baseline=cpu, stages=stages)
     print(json.dumps(out), flush=True)
 

@@ -324,7 +324,9 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
         const float* hp = hp_t + pph * NWP;              // the row's pulse taps (uniform)
         const float2* rl = rotw + (b * CB + c_l) * RW;   // this lane's child's rotations for the row
         const float2 dr = rl[4 * JT];
-        const float drr = dr.x, dri = dr.y;
+        // the rotation's imaginary part carried with the lane's sign (+ for the Im lane, - for the Re lane): the
+        // recurrence keeps that sign when its step's imaginary part is signed the same way
+        const float drr = dr.x, dris = t ? dr.y : -dr.y;
         #pragma unroll
         for (int kt = 0; kt < TPW; ++kt) {
           if (tact[kt]) {
@@ -352,16 +354,16 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
             // rho_c(p, j) at the lane's outputs j0 .. j0 + 3: the lane pair (t = 0, 1) holds (Re, Im); output
             // m - NWP + 1 + s of the pulse filter takes tap Dp (NWP - 1 - s) + ph of this row
             const float2 e0 = rl[4 * (wv + kt * NWV) + grp];
-            float er = e0.x, ei = e0.y;
+            float er = e0.x, eis = t ? e0.y : -e0.y;
             #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const float mine = acc[i];
                 const float other = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, mine), 0xB1, 0xF, 0xF, false));
-                const float v = fmaf(mine, er, other * (t ? ei : -ei));
+                const float v = fmaf(mine, er, other * eis);
                 #pragma unroll
                 for (int sl2 = 0; sl2 < NWP; ++sl2) acc_p[kt][sl2][i] = fmaf(hp[sl2], v, acc_p[kt][sl2][i]);
-                const float nr = fmaf(er, drr, -(ei * dri));
-                ei = fmaf(er, dri, ei * drr);
+                const float nr = fmaf(er, drr, -(eis * dris));
+                eis = fmaf(er, dris, eis * drr);
                 er = nr;
             }
           }
@@ -394,7 +396,7 @@ static int tap_pieces(int Dk, int NWp, int nthr)
 // the pulse filter's window length as instantiated (its polyphase table is built for this length)
 int level_mfma_nw(int Lp, int Dp)
 {
-    static const int set[] = {4, 6, 8, 10, 12, 16};
+    static const int set[] = {4, 6, 8, 9, 10, 12, 16};
     const int nw = (Lp + Dp - 1) / Dp;
     for (int v : set)
         if (v >= nw) return v;
@@ -488,7 +490,8 @@ int launch_level_mfma(const float2* x, int Np, int P, int K, const float* c0, co
 #define FSH_LM_P(A)                                                                                                   \
     switch (nwp) {                                                                                                    \
         case 4: FSH_LM_T(A, 4); break; case 6: FSH_LM_T(A, 6); break; case 8: FSH_LM_T(A, 8); break;                  \
-        case 10: FSH_LM_T(A, 10); break; case 12: FSH_LM_T(A, 12); break; default: FSH_LM_T(A, 16); break;            \
+        case 9: FSH_LM_T(A, 9); break; case 10: FSH_LM_T(A, 10); break; case 12: FSH_LM_T(A, 12); break;              \
+        default: FSH_LM_T(A, 16); break;                                                                              \
     }
     switch (KS) {
         case 1: FSH_LM_P(1); break; case 2: FSH_LM_P(2); break; case 3: FSH_LM_P(3); break; case 4: FSH_LM_P(4); break;

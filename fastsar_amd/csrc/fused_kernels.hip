@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void k_level_fused(const float2* __restrict__ 
 // the window lengths instantiated; a filter runs in the smallest one that holds ceil(L / D) (its table is zero padded)
 int fused_nw(int L, int D)
 {
-    static const int set[] = {1, 2, 3, 4, 6, 8, 10, 12, 16};
+    static const int set[] = {1, 2, 3, 4, 6, 8, 9, 10, 12, 16};
     const int nw = (L + D - 1) / D;
     for (int v : set)
         if (v >= nw) return v;
@@ -316,12 +316,14 @@ int launch_level_fused(const float2* x, int Np, int P, int K, const float* c0, c
     switch (NWP) {                                                                                                    \
         case 1: FSH_LF(A, 1); break;  case 2: FSH_LF(A, 2); break;  case 3: FSH_LF(A, 3); break;                     \
         case 4: FSH_LF(A, 4); break;  case 6: FSH_LF(A, 6); break;  case 8: FSH_LF(A, 8); break;                     \
-        case 10: FSH_LF(A, 10); break; case 12: FSH_LF(A, 12); break; default: FSH_LF(A, 16); break;                  \
+        case 9: FSH_LF(A, 9); break;  case 10: FSH_LF(A, 10); break; case 12: FSH_LF(A, 12); break;                  \
+        default: FSH_LF(A, 16); break;                                                                                \
     }
     switch (NWK) {
         case 1: FSH_LF_P(1); break;   case 2: FSH_LF_P(2); break;   case 3: FSH_LF_P(3); break;
         case 4: FSH_LF_P(4); break;   case 6: FSH_LF_P(6); break;   case 8: FSH_LF_P(8); break;
-        case 10: FSH_LF_P(10); break; case 12: FSH_LF_P(12); break; default: FSH_LF_P(16); break;
+        case 9: FSH_LF_P(9); break;   case 10: FSH_LF_P(10); break; case 12: FSH_LF_P(12); break;
+        default: FSH_LF_P(16); break;
     }
 #undef FSH_LF_P
 #undef FSH_LF

@@ -452,6 +452,7 @@ def main():
     former = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T, levels=args.levels or None,
                          atten=args.atten, taps=taps, precision=args.precision)
     plan = former._plan
+    form_T, form_levels, form_pred = former.T, former.levels, former.predicted_error_db
     log(f'former built in {time.perf_counter() - t0:.2f} s: T={former.T}, levels={former.levels}, '
         f'splits={plan["split"]}, final {plan["final"]["P"]} x {plan["final"]["K"]}, predicted {former.predicted_error_db:.1f} dB')
     for l, lv in enumerate(plan['levels']):
@@ -602,11 +603,8 @@ def main():
     if not args.no_batch and not args.small and args.history == 'complex64':
         # the 48-image batch in its bounded form (no candidate timing); bench.py --batch autotunes it. The image's
         # former and its buffers are released first.
-        fT, flev, fpred = former.T, former.levels, former.predicted_error_db
-        del himg, img
-        former = None
+        del himg, img, former
         bl = run_batch(args, tune=False)
-        former = type('FormerFacts', (), dict(T=fT, levels=flev, predicted_error_db=fpred))
         batch = dict(bl['config'], ms=bl['ms_per_step'], value=bl['value'], metric=bl['metric'])
 
     cpu = None
@@ -624,12 +622,12 @@ def main():
                warmup=args.warmup, ms_per_step=round(dt * 1e3, 2), higher_is_better=True, scaling='weak',
                vs_baseline=None, dtype='f32', data='synthetic',
                config=dict(workload=(f'{"512x512 from 512x1024" if args.small else "12288x8192 image from 16384 pulses x 8192 samples"}'
-                                     f' {args.history}, factorized backprojection, T={former.T}, {former.levels} levels, '
+                                     f' {args.history}, factorized backprojection, T={form_T}, {form_levels} levels, '
                                      + (f'learned filters of {taps.T} taps per phase' if taps else f'{args.atten:g} dB filters')),
                            history=args.history, precision=args.precision,
                            s_per_image=round(dt, 4), device_median_ms=round(float(np.median(step_s)) * 1e3, 2),
                            error_db=round(err, 2), error_tiles=f'{len(tiles)} x {side}^2',
-                           predicted_error_db=round(former.predicted_error_db, 2), parallelism='single GPU',
+                           predicted_error_db=round(form_pred, 2), parallelism='single GPU',
                            host_to_host_ms=None if h2h is None else round(h2h * 1e3, 1),
                            host_to_host_steps=args.host_steps if h2h is not None else 0, learned_form=learned_form,
                            exact_form=exact, batch_form=batch),

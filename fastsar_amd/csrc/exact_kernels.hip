@@ -579,6 +579,14 @@ int fsh_exact_create(const fsh_exact_desc* desc, void** handle)
         set_error("exact_create: invalid descriptor");
         return E_INVALID;
     }
+    {
+        BpShape sh;
+        if (!bp_shape(d, sh)) {
+            set_error("exact_create: a pixel region's range window does not fit in LDS (use a smaller upsample or pixel "
+                      "spacing)");
+            return E_INVALID;
+        }
+    }
     Exact* e = new Exact();
     e->d = d;
     {

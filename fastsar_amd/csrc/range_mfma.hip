@@ -33,19 +33,8 @@ typedef unsigned int uint4_t __attribute__((ext_vector_type(4)));
 
 constexpr int STG = 16 * 66;               // per-wave transpose buffer (floats): 16 children x 32 outputs x (Re, Im)
 
-__device__ __forceinline__ unsigned int pack2(__bf16 lo, __bf16 hi)
-{
-    return (unsigned int)__builtin_bit_cast(unsigned short, lo) | ((unsigned int)__builtin_bit_cast(unsigned short, hi) << 16);
-}
-
-__device__ __forceinline__ void split(float v, __bf16& hi, __bf16& lo)
-{
-    hi = (__bf16)v;
-    lo = (__bf16)(v - (float)hi);
-}
-
-// split of two values at once into packed words: hi = (bf16 a, bf16 b), lo = the bf16 of the remainders (the same
-// roundings as split)
+// split of two values at once into packed words: hi = (bf16 a, bf16 b), lo = the bf16 of the remainders a - hi,
+// b - hi (each rounded to nearest even)
 __device__ __forceinline__ void split_pair(float a, float b, unsigned int& hi, unsigned int& lo)
 {
     typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
@@ -77,7 +66,7 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
     unsigned int* XL = XH + NPH * PS;
     float* stg_all = reinterpret_cast<float*>(XL + NPH * PS);  // [4][STG]
     PairTab* tab = reinterpret_cast<PairTab*>(stg_all + 4 * STG);   // [4 KS]
-    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x, wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int col = lane & 31, q = lane >> 5;
     const int p = blockIdx.x, n = blockIdx.y;
     const int MT = (Ko + 31) / 32;
@@ -96,33 +85,40 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
         tab[i] = e;
     }
     const float2* xrow = x + ((size_t)n * P + p) * K;
-    // the window words i = tid + 256 k (word m of plane ph), XB loads issued before the first is split and stored
+    // the window: samples ks + i for i < D WM, read in order (coalesced); sample i is word m = i / D of plane
+    // ph = i mod D, (m, ph) stepped by 256 samples at a time; XB loads issued before the first is split and stored
     {
-        constexpr int XB = 8;
-        const int total = NPH * WM;
-        int ph = tid / WM, m = tid - ph * WM;
+        constexpr int XB = 6;
+        const int total = D * WM, dm = 256 / D, dph = 256 - dm * D;
+        int m = tid / D, ph = tid - m * D;
         for (int i0 = tid; i0 < total; i0 += 256 * XB) {
             float2 v[XB];
             int dst[XB];
             #pragma unroll
             for (int k = 0; k < XB; ++k) {
-                const int i = i0 + 256 * k, kk = ks + D * m + ph;
+                const int i = i0 + 256 * k, kk = ks + i;
                 v[k] = make_float2(0.f, 0.f);
-                if (i < total && ph < D && kk >= 0 && kk < K) v[k] = xrow[kk];
+                if (i < total && kk >= 0 && kk < K) v[k] = xrow[kk];
                 dst[k] = i < total ? ph * PS + m : -1;
-                m += 256;
-                while (m >= WM) { m -= WM; ++ph; }
+                m += dm;
+                ph += dph;
+                if (ph >= D) { ph -= D; ++m; }
             }
             #pragma unroll
             for (int k = 0; k < XB; ++k) {
                 if (dst[k] >= 0) {
-                    __bf16 rh, rl, ih, il;
-                    split(v[k].x, rh, rl);
-                    split(v[k].y, ih, il);
-                    XH[dst[k]] = pack2(rh, ih);
-                    XL[dst[k]] = pack2(rl, il);
+                    unsigned int hi, lo;
+                    split_pair(v[k].x, v[k].y, hi, lo);
+                    XH[dst[k]] = hi;
+                    XL[dst[k]] = lo;
                 }
             }
+        }
+        // planes D .. NPH - 1 carry the zero taps' samples
+        for (int i = tid; i < (NPH - D) * WM; i += 256) {
+            const int pz = D + i / WM, mz = i - (pz - D) * WM;
+            XH[pz * PS + mz] = 0u;
+            XL[pz * PS + mz] = 0u;
         }
     }
     __syncthreads();
@@ -132,6 +128,9 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
     const int wpn = NT >= 4 ? 1 : 4 / NT;                      // waves per N-tile
     const int nt_first = wv / wpn, ws = wv - nt_first * wpn;
     const int nt_step = 4 / wpn;
+    // this wave's M-tiles: a contiguous share [ma, mb) of the chunk's, so that tile k of a run lies 32 k words on
+    const int mper = (m1 - m0 + wpn - 1) / wpn;
+    const int ma = min(m1, m0 + ws * mper), mb = min(m1, ma + mper);
     int off[KS][2];
     #pragma unroll
     for (int s = 0; s < KS; ++s) {
@@ -148,7 +147,9 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
             b = (double)sl[((size_t)n * Cn + c) * P + p];
         }
         // the N-tile's taps: B[kappa][column] for this lane's column (child c, Re or Im) and contraction indices
-        // 8s + 4q .. 8s + 4q + 3; rows (Re x, Im x) of column Re y_c hold (Re g, -Im g), of column Im y_c (Im g, Re g)
+        // 8s + 4q .. 8s + 4q + 3; rows (Re x, Im x) of column Re y_c hold (Re g, -Im g), of column Im y_c (Im g, Re g).
+        // The lane pair (t = 0, 1) of a child shares its taps: lane t evaluates the pair of taps pr = t and takes
+        // pair 1 - t from its partner.
         uint4_t Gw[KS], Gv[KS];                                 // hi and lo operand words
         {
             float vdr, vdi;
@@ -158,16 +159,23 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
             asm volatile("" : "+s"(tb));                       // the table reads stay here, not hoisted
             #pragma unroll
             for (int s = 0; s < KS; ++s) {
+                const PairTab e = tab[tb + 4 * s + 2 * q + t];
+                // exp(2 pi i sl r0): the product's rounding error recovered by an fma, the cycle wrapped exactly
+                const float pp = slf * e.r0, pe = fmaf(slf, e.r0, -pp);
+                const float fr = (pp - rintf(pp)) + pe;
+                const float sr = __builtin_amdgcn_cosf(fr), si = __builtin_amdgcn_sinf(fr);
+                const float s2r = fmaf(sr, vdr, -(si * vdi)), s2i = fmaf(sr, vdi, si * vdr);
+                const float h0 = cok ? e.h0 : 0.f, h1 = cok ? e.h1 : 0.f;
+                float g[4] = {h0 * sr, h0 * si, h1 * s2r, h1 * s2i}, o[4];
+                #pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    o[i] = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, g[i]), 0xB1, 0xF, 0xF, false));
                 #pragma unroll
                 for (int pr = 0; pr < 2; ++pr) {
-                    const PairTab e = tab[tb + 4 * s + 2 * q + pr];
-                    // exp(2 pi i sl r0): the product's rounding error recovered by an fma, the cycle wrapped exactly
-                    const float pp = slf * e.r0, pe = fmaf(slf, e.r0, -pp);
-                    const float fr = (pp - rintf(pp)) + pe;
-                    const float sr = __builtin_amdgcn_cosf(fr), si = __builtin_amdgcn_sinf(fr);
-                    const float s2r = fmaf(sr, vdr, -(si * vdi)), s2i = fmaf(sr, vdi, si * vdr);
-                    const float h0 = cok ? e.h0 : 0.f, h1 = cok ? e.h1 : 0.f;
-                    const float g0r = h0 * sr, g0i = h0 * si, g1r = h1 * s2r, g1i = h1 * s2i;
+                    // (g0r, g0i, g1r, g1i) of pair pr: this lane's own where pr = t, the partner's otherwise
+                    const bool own = pr == 0 ? !t : t;
+                    const float g0r = own ? g[0] : o[0], g0i = own ? g[1] : o[1];
+                    const float g1r = own ? g[2] : o[2], g1i = own ? g[3] : o[3];
                     const float a0 = t ? g0i : g0r, b0 = t ? g0r : -g0i;
                     const float a1 = t ? g1i : g1r, b1 = t ? g1r : -g1i;
                     unsigned int h0w, l0w, h1w, l1w;
@@ -189,66 +197,87 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
             cis_cyc(b * (double)D, sDr, sDi);
             cis_cyc(b * (double)(8 * D), s8r, s8i);
         }
-        for (int mt = m0 + ws; mt < m1; mt += wpn) {
-            floatx16_t acc = {};
-            const int jb = (mt - m0) * 32 + col;
+        // outputs: child nt 16 + q + 2 it at row p, sample j, as base + it (2 P Ko) + lane offset (q P Ko + j)
+        const size_t ybase = (((size_t)n * Cn + nt * 16) * P + p) * Ko;
+        const size_t ystep = 2 * (size_t)P * Ko;
+        const unsigned int ylane = (unsigned int)q * (unsigned int)P * (unsigned int)Ko + (unsigned int)col;
+        // runs of up to 8 M-tiles: tile k of a run reads 32 k words past the run's first, an immediate offset
+        for (int mr = ma; mr < mb; mr += 8) {
+            const int cnt = min(8, mb - mr);
+            const int jr = (mr - m0) * 32 + col;
             #pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                const unsigned int* h0 = XH + off[s][0] + jb;
-                const unsigned int* h1 = XH + off[s][1] + jb;
-                const unsigned int* l0 = XL + off[s][0] + jb;
-                const unsigned int* l1 = XL + off[s][1] + jb;
-                uint4_t wh, wl;
-                wh.x = h0[0]; wh.y = h0[1]; wh.z = h1[0]; wh.w = h1[1];
-                wl.x = l0[0]; wl.y = l0[1]; wl.z = l1[0]; wl.w = l1[1];
-                const bf16x8_t Ah = __builtin_bit_cast(bf16x8_t, wh), Al = __builtin_bit_cast(bf16x8_t, wl);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gh[s], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gl[s], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Gh[s], acc, 0, 0, 0);
-            }
-            // rho at rows 4q, 4q + 1, ... of this tile: seeded at row 4q, stepped by D samples, 8 rows apart by 8 D;
-            // the lane pair (t = 0, 1) holds (Re, Im) of one output: Re' = Re er - Im ei, Im' = Im er + Re ei
-            if (!ROT) {
-                #pragma unroll
-                for (int i = 0; i < 16; ++i) stg[(col >> 1) * 66 + 2 * (8 * (i >> 2) + 4 * q + (i & 3)) + t] = acc[i];
-            }
-            float rr = 1.f, ri = 0.f;
-            if (ROT) cis_cyc(fma((double)(D * (mt * 32 + 4 * q) - pl) - kc, b, a), rr, ri);
-            #pragma unroll
-            for (int g = 0; g < (ROT ? 4 : 0); ++g) {
-                float er = rr, ei = ri;
-                #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int i = 4 * g + e;
-                    const float mine = acc[i];
-                    const float other = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, mine), 0xB1, 0xF, 0xF, false));
-                    stg[(col >> 1) * 66 + 2 * (8 * g + 4 * q + e) + t] = fmaf(mine, er, other * (t ? ei : -ei));
-                    const float nr = fmaf(er, sDr, -(ei * sDi));
-                    ei = fmaf(er, sDi, ei * sDr);
-                    er = nr;
-                }
-                const float nr = fmaf(rr, s8r, -(ri * s8i));
-                ri = fmaf(rr, s8i, ri * s8r);
-                rr = nr;
-            }
-            wave_lds_sync();
-            const int j = mt * 32 + col;
-            #pragma unroll
-            for (int it = 0; it < 8; ++it) {
-                const int cl = q + 2 * it, cc = nt * 16 + cl;
-                if (cc < Cn && j < Ko) {
-                    const float2 v = *reinterpret_cast<const float2*>(stg + cl * 66 + 2 * col);
-                    const size_t o = (((size_t)n * Cn + cc) * P + p) * Ko + j;
+            for (int k = 0; k < 8; ++k) {
+                if (k < cnt) {
+                    const int mt = mr + k;
+                    floatx16_t acc = {};
+                    #pragma unroll
+                    for (int s = 0; s < KS; ++s) {
+                        const unsigned int* h0 = XH + off[s][0] + jr + 32 * k;
+                        const unsigned int* h1 = XH + off[s][1] + jr + 32 * k;
+                        const unsigned int* l0 = XL + off[s][0] + jr + 32 * k;
+                        const unsigned int* l1 = XL + off[s][1] + jr + 32 * k;
+                        uint4_t wh, wl;
+                        wh.x = h0[0]; wh.y = h0[1]; wh.z = h1[0]; wh.w = h1[1];
+                        wl.x = l0[0]; wl.y = l0[1]; wl.z = l1[0]; wl.w = l1[1];
+                        const bf16x8_t Ah = __builtin_bit_cast(bf16x8_t, wh), Al = __builtin_bit_cast(bf16x8_t, wl);
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gh[s], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gl[s], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Gh[s], acc, 0, 0, 0);
+                    }
+                    // rho at rows 4q, 4q + 1, ... of this tile: seeded at row 4q, stepped by D samples, 8 rows apart
+                    // by 8 D; the lane pair (t = 0, 1) holds (Re, Im) of one output: Re' = Re er - Im ei,
+                    // Im' = Im er + Re ei
+                    if (!ROT) {
+                        #pragma unroll
+                        for (int i = 0; i < 16; ++i) stg[(col >> 1) * 66 + 2 * (8 * (i >> 2) + 4 * q + (i & 3)) + t] = acc[i];
+                    }
+                    float rr = 1.f, ri = 0.f;
+                    if (ROT) cis_cyc(fma((double)(D * (mt * 32 + 4 * q) - pl) - kc, b, a), rr, ri);
+                    #pragma unroll
+                    for (int g = 0; g < (ROT ? 4 : 0); ++g) {
+                        float er = rr, ei = ri;
+                        #pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int i = 4 * g + e;
+                            const float mine = acc[i];
+                            const float other = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, mine), 0xB1, 0xF, 0xF, false));
+                            stg[(col >> 1) * 66 + 2 * (8 * g + 4 * q + e) + t] = fmaf(mine, er, other * (t ? ei : -ei));
+                            const float nr = fmaf(er, sDr, -(ei * sDi));
+                            ei = fmaf(er, sDi, ei * sDr);
+                            er = nr;
+                        }
+                        const float nr = fmaf(rr, s8r, -(ri * s8i));
+                        ri = fmaf(rr, s8i, ri * s8r);
+                        rr = nr;
+                    }
+                    wave_lds_sync();
+                    // a full tile (every child and sample of it inside the output) stores without lane masks
+                    const bool full = nt * 16 + 16 <= Cn && mt * 32 + 32 <= Ko;
+                    const bool jok = mt * 32 + col < Ko;
+                    const unsigned int yo = ylane + (unsigned int)(mt * 32);
                     if (hs) {                          // float16 storage, scaled by hs[0] (a power of two)
                         typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-                        const float g = hs[0];
-                        reinterpret_cast<h2*>(y)[o] = h2{(_Float16)(v.x * g), (_Float16)(v.y * g)};
+                        const float gs = hs[0];
+                        #pragma unroll
+                        for (int it = 0; it < 8; ++it) {
+                            if (full || (nt * 16 + 2 * it < Cn - q && jok)) {
+                                const float2 v = *reinterpret_cast<const float2*>(stg + (q + 2 * it) * 66 + 2 * col);
+                                (reinterpret_cast<h2*>(y) + ybase + it * ystep)[yo] = h2{(_Float16)(v.x * gs), (_Float16)(v.y * gs)};
+                            }
+                        }
+                    } else if (full) {
+                        #pragma unroll
+                        for (int it = 0; it < 8; ++it)
+                            (y + ybase + it * ystep)[yo] = *reinterpret_cast<const float2*>(stg + (q + 2 * it) * 66 + 2 * col);
                     } else {
-                        y[o] = v;
+                        #pragma unroll
+                        for (int it = 0; it < 8; ++it)
+                            if (nt * 16 + 2 * it < Cn - q && jok)
+                                (y + ybase + it * ystep)[yo] = *reinterpret_cast<const float2*>(stg + (q + 2 * it) * 66 + 2 * col);
                     }
+                    wave_lds_sync();                   // the next tile's writes to stg come after these reads
                 }
             }
-            wave_lds_sync();                                   // the next tile's writes to stg come after these reads
         }
     }
 }
@@ -283,6 +312,10 @@ int launch_rot_fir_k_mf2(const float2* x, int Np, int P, int K, const float* c0,
     int NWp, KS, NPH, PS, MC, chunks;
     size_t lds;
     if (!mf2_shape(L, D, Ko, NWp, KS, NPH, PS, MC, chunks, lds)) return E_INVALID;
+    if ((size_t)P * Ko >= ((size_t)1 << 31)) {                // the stores' lane offsets are 32-bit
+        set_error("rot_fir_k_mf2: rows of more than 2^31 outputs a child");
+        return E_INVALID;
+    }
     for (int n0 = 0; n0 < Np; n0 += 65535) {
         const int nn = std::min(65535, Np - n0);
         const dim3 grid((unsigned)P, (unsigned)nn, (unsigned)chunks);

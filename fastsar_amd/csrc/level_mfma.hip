@@ -12,9 +12,10 @@
 //     indices as adjacent word pairs and a half-wave reads consecutive words);
 //   * the children's complex taps for the row are built once into LDS in the operand layout of
 //     v_mfma_f32_16x16x32_bf16 (each lane then reads its operand as two 16-byte words, hi and lo in separate
-//     64-lane planes so that a wave's reads are conflict-free): a thread takes a run of taps of one child and one
-//     phase, r = Dk u + ph for u in [u0, u1), and steps exp(2 pi i sl r) along it by the factor exp(2 pi i sl Dk),
-//     two sine-cosine pairs per run instead of one per tap;
+//     64-lane planes so that a wave's reads are conflict-free): tap r = Dk u + ph of child c is
+//     hk[r] E_c[ph] F_c[u] with E_c[ph] = exp(2 pi i sl_c ph) and F_c[u] = exp(2 pi i sl_c Dk u), one complex
+//     product per tap from a per-row table of Dk + ceil(Lk / Dk) phasors a child, built a row ahead (its own
+//     double buffer), so that no thread evaluates a sine or cosine for a tap;
 //   * each wave multiplies its 16 outputs by the 8 children (16 columns: Re and Im of each child), rotates the
 //     result by rho and adds it into the NWP sliding accumulators of the pulse filter, which hold its 4 values per
 //     lane for the NWP outputs the row feeds; a finished output is stored.
@@ -73,9 +74,11 @@ __device__ __forceinline__ void split2(float a, float b, unsigned int& hi, unsig
     lo = __builtin_bit_cast(unsigned int, l);
 }
 
-constexpr int MAXG = 2;                    // tap runs built per thread and row (at most)
 constexpr int WAVES_MULTI = 8;             // waves of a block whose waves take several tiles
-constexpr int TAB_MAX = 128;               // (phase, u) tap positions: Dk NWp <= 4 ceil(Dk / 4) NWp = 8 (2 KS) <= 128
+constexpr int TAB_MAX = 128;               // taps: Lk <= Dk NWp <= 4 ceil(Dk / 4) NWp = 8 (2 KS) <= 128
+
+// phasor-table entries (child, slot) computed per thread and row (at most), by block size
+__host__ __device__ constexpr int ef_items(int tpw) { return tpw == 1 ? 3 : 2; }
 
 }  // namespace
 
@@ -85,8 +88,7 @@ template <int KS, int NWP, int TPW>
 __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1 && NWP <= 10 ? 4 : 1) void k_level_mfma(
     const float2* __restrict__ x, int P, int K, const float* __restrict__ c0, const float* __restrict__ sl, int C,
     const float* __restrict__ hk, int Lk, int Dk, int plk, int Ko, int NWp, int NPH, int PS, int JT, int njr,
-    const float* __restrict__ hp_t, int Dp, int plp, int Lp, int Po, int IS, int nseg, float kcf, int npc,
-    float2* __restrict__ z)
+    const float* __restrict__ hp_t, int Dp, int plp, int Lp, int Po, int IS, int nseg, float kcf, float2* __restrict__ z)
 {
     constexpr int MAXI = TPW == 1 ? 4 : 8;             // window words staged per thread and row (at most)
     const int NWV = blockDim.x >> 6;                   // waves: wave w takes tiles w, w + NWV, ... of the block's JT
@@ -111,14 +113,15 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
     const int WM = 16 * JT + NWp;                        // plane words used
 
     // LDS: planes [2][2 (hi, lo)][NPH][PS] words, tap fragments [2][KS][2 (hi, lo)][64] uint4, the A operand's pair
-    // offsets [8 KS], the taps' fragment words and values [TAB_MAX] each
+    // offsets [8 KS], the taps' table [TAB_MAX], the rotations [2][CB][RW] and the phasor tables [2][CB][NE]
     unsigned int* planes = lds_u;
     uint4_t* frag = reinterpret_cast<uint4_t*>(planes + 4 * NPH * PS);
     int* taba = reinterpret_cast<int*>(frag + 2 * KS * 2 * 64);  // [8 KS] pair offsets ph PS + u
-    int* tabw = taba + 8 * KS;                                   // [Dk NWp] word of tap (ph, u), child 0, Re column
-    float* tabh = reinterpret_cast<float*>(tabw + TAB_MAX);     // [Dk NWp] its real tap value
+    int4* tapt = reinterpret_cast<int4*>(taba + 8 * KS);        // [Lk] taps: byte offsets of word, E[ph], F[u]; value
     const int RW = 4 * JT + 1;
-    float2* rotw = reinterpret_cast<float2*>(tabh + TAB_MAX);   // [2][CB][RW]: rho at each lane group's first output, step
+    float2* rotw = reinterpret_cast<float2*>(tapt + TAB_MAX);   // [2][CB][RW]: rho at each lane group's first output, step
+    const int NU = (Lk + Dk - 1) / Dk, NE = Dk + NU;
+    float2* ef = rotw + 2 * CB * RW;                             // [2][CB][NE]: E_c[ph] (ph < Dk), then F_c[u]
 
     // contraction slot sigma = 16 s + 4 g + ew (k-step s, lane group g, word ew = 2 pr + e of the lane's operand)
     // holds tap (ph, u) = (4 pb + g, 2 up + e), where pr-pair column 2 s + pr = pb (NWp / 2) + up: the four lane groups
@@ -129,13 +132,16 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
         const int col = 2 * st + pr, pb = col / (NWp >> 1), up = col - pb * (NWp >> 1);
         taba[i] = (4 * pb + g) * PS + 2 * up;
     }
-    // tap (ph, u), ph < Dk, u < NWp: its word in a buffer's hi plane at lane 16 g (child 0, Re column) and its value;
-    // positions past the filter (r >= Lk) keep the zeros written below
-    for (int i = tid; i < Dk * NWp; i += nthr) {
-        const int ph = i / NWp, u = i - ph * NWp, r = Dk * u + ph;
+    // the taps r = Dk u + ph < Lk in the order (ph, u), u fastest: entry i holds the byte offsets of its word in a
+    // buffer's hi plane at lane 16 g (child 0, Re column) and of E[ph] and F[u] in a child's phasor table, and its
+    // value (positions past the filter keep the zeros written below). In this order the eight entries a wave takes
+    // at once cover the four words ew of an operand, so its writes spread over 32 banks (64 with the Im columns)
+    for (int i = tid; i < Lk; i += nthr) {
+        int ph = 0, i0 = 0;
+        while (i - i0 >= (Lk - ph + Dk - 1) / Dk) i0 += (Lk - ph++ + Dk - 1) / Dk;
+        const int u = i - i0;
         const int g = ph & 3, col = (ph >> 2) * (NWp >> 1) + (u >> 1), st = col >> 1, ew = 2 * (col & 1) + (u & 1);
-        tabw[i] = r < Lk ? ((2 * st) * 64 + 16 * g) * 4 + ew : -1;
-        tabh[i] = r < Lk ? hk[r] : 0.f;
+        tapt[i] = make_int4(4 * (((2 * st) * 64 + 16 * g) * 4 + ew), 8 * ph, 8 * (Dk + u), __float_as_int(hk[Dk * u + ph]));
     }
     {
         unsigned int* fw = reinterpret_cast<unsigned int*>(frag);
@@ -147,8 +153,7 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
     // Staging, with the global loads of row p + 1 issued before row p computes and written to LDS after it:
     //  * plane items idx = tid + k nthr (k < MAXI) of the NPH x WM window words: word m of plane ph, sample
     //    Dk (jbase + m) + ph - plk (the same for every row);
-    //  * tap entries i = tid + k nthr (k < MAXG) of the 16 KS x CB (contraction index, child) table: child tid mod 8
-    //    for every k (nthr is a multiple of 8), so each thread reads one slope per row.
+    //  * phasor-table entries i = tid + k nthr (k < MAXE) of the CB x NE table, for the row after next.
     int it_lds[MAXI], it_k[MAXI];
     #pragma unroll
     for (int k = 0; k < MAXI; ++k) {
@@ -163,20 +168,22 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
         #pragma unroll
         for (int hb = 0; hb < 4; ++hb) planes[(size_t)hb * NPH * PS + Dk * PS + i] = 0u;
     }
-    // tap runs w = tid + k nthr of the CB Dk (child, phase) runs cut into npc pieces: run q = ph CB + c, piece
-    // w mod npc takes u in [u0, u1) (the u with r = Dk u + ph < Lk); children past C keep their zero taps
-    const int ulen = (NWp + npc - 1) / npc;
-    int g_c[MAXG], g_ph[MAXG], g_u0[MAXG], g_u1[MAXG];
+    // phasor items i = tid + k nthr of the CB x NE table: child c = i / NE, slot e = i mod NE: E_c[e] =
+    // exp(2 pi i sl_c e) for e < Dk, F_c[e - Dk] = exp(2 pi i sl_c Dk (e - Dk)) after; children past C are not built
+    // (their taps stay zero)
+    constexpr int MAXE = ef_items(TPW);
+    // taps: thread tid builds child tc's, numbered from wave 1 (tt) so that the wave taking a row's leftover taps is
+    // not the one taking the first phasor and rotation items
+    const int tt = (tid + nthr - (nthr > 64 ? 64 : 0)) % nthr, tc = tid & (CB - 1);
+    const bool tok = cb * CB + tc < C;
+    int e_i[MAXE];
+    float e_m[MAXE], e_s[MAXE];
     #pragma unroll
-    for (int k = 0; k < MAXG; ++k) {
-        const int w = tid + k * nthr, q = w / npc, pc = w - q * npc;
-        const int c = q % CB, ph = q / CB;
-        const int nu = ph < Dk ? (Lk - ph + Dk - 1) / Dk : 0;       // taps of this phase inside the filter
-        const bool ok = q < CB * Dk && cb * CB + c < C;
-        g_c[k] = ok ? c : -1;
-        g_ph[k] = ph;
-        g_u0[k] = pc * ulen;
-        g_u1[k] = min(nu, pc * ulen + ulen);
+    for (int k = 0; k < MAXE; ++k) {
+        const int i = tid + k * nthr, c = i / NE, e = i - c * NE;
+        e_i[k] = (i < CB * NE && cb * CB + c < C) ? i : -1;
+        e_m[k] = (float)(e < Dk ? e : Dk * (e - Dk));
+        e_s[k] = 0.f;
     }
     // rotation items i = tid + k nthr of the CB x RW table: child c = i / RW; s = i mod RW < 4 JT: rho_c(p, j) at the
     // first output j = 16 (jr JT + s / 4) + 4 (s mod 4) of a lane group, exp(2 pi i (c0 + (Dk j - plk - kc) sl));
@@ -186,20 +193,32 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
     float r_m[MAXR], r_a[MAXR], r_b[MAXR];
     #pragma unroll
     for (int k = 0; k < MAXR; ++k) {
-        const int i = tid + k * nthr, c = i / RW, si = i - c * RW;
+        const int i = (tid + nthr / 2) % nthr + k * nthr, c = i / RW, si = i - c * RW;   // from the middle wave on
         r_i[k] = i < CB * RW ? i : -1;
         r_m[k] = si < 4 * JT ? (float)(Dk * (16 * (jr * JT + (si >> 2)) + 4 * (si & 3)) - plk) - kcf : (float)Dk;
         r_a[k] = r_b[k] = 0.f;
     }
     float2 xv[MAXI];
-    float gsl[MAXG];
+    // the phasor tables of row p: slopes loaded (load_ef), then the phasors written to buffer eb (build_ef)
+    auto load_ef = [&](int p) {
+        #pragma unroll
+        for (int k = 0; k < MAXE; ++k)
+            if (e_i[k] >= 0) e_s[k] = sl[((size_t)n * C + cb * CB + e_i[k] / NE) * P + p];
+    };
+    auto build_ef = [&](int eb) {
+        #pragma unroll
+        for (int k = 0; k < MAXE; ++k) {
+            if (e_i[k] >= 0) {
+                float cs, sn;
+                cis_lin(0.f, e_s[k], e_m[k], cs, sn);
+                ef[eb * CB * NE + e_i[k]] = make_float2(cs, sn);
+            }
+        }
+    };
     auto load_row = [&](int p) {
         const float2* xr = xn + (size_t)p * K;
         #pragma unroll
         for (int k = 0; k < MAXI; ++k) xv[k] = it_k[k] >= 0 ? xr[it_k[k]] : make_float2(0.f, 0.f);
-        #pragma unroll
-        for (int k = 0; k < MAXG; ++k)
-            gsl[k] = (g_c[k] >= 0 && g_u0[k] < g_u1[k]) ? sl[((size_t)n * C + cb * CB + g_c[k]) * P + p] : 0.f;
         #pragma unroll
         for (int k = 0; k < MAXR; ++k) {
             const int c = r_i[k] / RW, cgl = cb * CB + c;
@@ -222,32 +241,25 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
                 XL[it_lds[k]] = lo;
             }
         }
-        unsigned int* FH = reinterpret_cast<unsigned int*>(frag + (size_t)b * KS * 2 * 64);
-        #pragma unroll
-        for (int k = 0; k < MAXG; ++k) {
-            if (g_c[k] >= 0 && g_u0[k] < g_u1[k]) {
-                // exp(2 pi i sl r) at r = Dk u0 + ph, then times exp(2 pi i sl Dk) per step of u
-                float cs, sn, dc, ds;
-                cis_lin(0.f, gsl[k], (float)(Dk * g_u0[k] + g_ph[k]), cs, sn);
-                cis_lin(0.f, gsl[k], (float)Dk, dc, ds);
-                const int* tw = tabw + g_ph[k] * NWp;
-                const float* th = tabh + g_ph[k] * NWp;
-                const int cw = 8 * g_c[k];                     // lane 2 c: the child's Re column
-                #pragma unroll 1
-                for (int u = g_u0[k]; u < g_u1[k]; ++u) {
-                    const float h = th[u];
-                    const int wd = tw[u] + cw;
-                    unsigned int hi, lo;
-                    split2(h * cs, -(h * sn), hi, lo);         // column Re y_c: (Re g, -Im g)
-                    FH[wd] = hi;
-                    FH[wd + 256] = lo;
-                    // column Im y_c (lane 2 c + 1): (Im g, Re g), the same halves swapped with the sign of Im restored
-                    FH[wd + 4] = __builtin_amdgcn_alignbit(hi, hi, 16) ^ 0x8000u;
-                    FH[wd + 260] = __builtin_amdgcn_alignbit(lo, lo, 16) ^ 0x8000u;
-                    const float nc = fmaf(cs, dc, -(sn * ds));
-                    sn = fmaf(cs, ds, sn * dc);
-                    cs = nc;
-                }
+        // table entries i = tt / 8 + k nthr / 8 of child tc: hk[r] E_c[ph] F_c[u] from the row's phasor table
+        // (buffer b, as the planes), written at the child's Re column (lane 2 c) and Im column (lane 2 c + 1)
+        if (tok) {
+            char* FH = reinterpret_cast<char*>(frag + (size_t)b * KS * 2 * 64) + 32 * tc;
+            const char* E = reinterpret_cast<const char*>(ef + (b * CB + tc) * NE);
+            #pragma unroll 1
+            for (int i = tt >> 3; i < Lk; i += nthr >> 3) {
+                const int4 e = tapt[i];
+                const float2 fa = *reinterpret_cast<const float2*>(E + e.y), fb = *reinterpret_cast<const float2*>(E + e.z);
+                const float h = __int_as_float(e.w);
+                const float gr = h * fmaf(fa.x, fb.x, -(fa.y * fb.y)), gi = h * fmaf(fa.x, fb.y, fa.y * fb.x);
+                unsigned int hi, lo;
+                split2(gr, -gi, hi, lo);                       // column Re y_c: (Re g, -Im g)
+                unsigned int* w = reinterpret_cast<unsigned int*>(FH + e.x);
+                w[0] = hi;
+                w[256] = lo;
+                // column Im y_c: (Im g, Re g), the same halves swapped with the sign of Im restored
+                w[4] = __builtin_amdgcn_alignbit(hi, hi, 16) ^ 0x8000u;
+                w[260] = __builtin_amdgcn_alignbit(lo, lo, 16) ^ 0x8000u;
             }
         }
         #pragma unroll
@@ -312,15 +324,23 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
     // rows before 0 (first segment): zero rows, only the period bookkeeping
     for (int p = pbeg; p < plo; ++p)
         if (++pph == Dp) { end_period(); pph = 0; }
+    // the phasor tables run a row ahead of the planes and taps: row p + 2's are built while row p computes, into the
+    // buffer row p's used
     if (plo < phi) {
+        load_ef(plo);
         load_row(plo);
+        build_ef(0);
+        if (plo + 1 < phi) load_ef(plo + 1);
+        __syncthreads();
         store_row(0);
+        if (plo + 1 < phi) build_ef(1);
     }
     __syncthreads();
     for (int p = plo; p < phi; ++p) {
         const int b = (p - plo) & 1;
         const bool more = p + 1 < phi;
         if (more) load_row(p + 1);
+        if (p + 2 < phi) load_ef(p + 2);
         const float* hp = hp_t + pph * NWP;              // the row's pulse taps (uniform)
         const float2* rl = rotw + (b * CB + c_l) * RW;   // this lane's child's rotations for the row
         const float2 dr = rl[4 * JT];
@@ -370,25 +390,12 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
         }
         if (++pph == Dp) { end_period(); pph = 0; }
         if (more) store_row(b ^ 1);
+        if (p + 2 < phi) build_ef(b);
         __syncthreads();
     }
     // rows past the last read: finish the open period, then every period until the segment's last output is stored
     if (pph != 0) { end_period(); pph = 0; }
     while (pm - NWP + 1 < i1) end_period();
-}
-
-// pieces each (child, phase) run of taps is cut into: runs of about 3 taps (two sine-cosine pairs per run against a
-// multiply per tap; on the 12288 x 8192 image's level 2, runs of 2-3 taps over all threads took 116.5 ms, of 5 taps
-// over half the threads 119.0 ms, of 10 taps 138.2 ms, one pair per tap 131 ms), fewer pieces where the block has
-// fewer threads than pieces
-static int tap_pieces(int Dk, int NWp, int nthr)
-{
-    int len = 3;
-    if (const char* e = getenv("FSH_LMFMA_TAPRUN")) len = std::max(1, atoi(e));    // sweeps
-    const int runs = CB * Dk;
-    int npc = std::max(1, (NWp + len - 1) / len);
-    while (npc > 1 && runs * npc > nthr) --npc;
-    return npc;
 }
 
 // the launch shape, or false where the form does not apply: both axes decimated, the range taps in 128 contraction
@@ -431,12 +438,13 @@ bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int
     }
     PS = 16 * JT + NWp;
     PS = ((PS - 16 + 63) / 64) * 64 + 16;                  // 16 mod 64 words: the lane groups' runs on disjoint banks
-    lds = (size_t)4 * 4 * NPH * PS + (size_t)2 * KS * 64 * 2 * 16 + (size_t)8 * KS * 4 + (size_t)2 * TAB_MAX * 4 +
-          (size_t)2 * CB * (4 * JT + 1) * 8;
+    const int NE = Dk + NW;
+    lds = (size_t)4 * 4 * NPH * PS + (size_t)2 * KS * 64 * 2 * 16 + (size_t)8 * KS * 4 + (size_t)TAB_MAX * 16 +
+          (size_t)2 * CB * (4 * JT + 1) * 8 + (size_t)2 * CB * NE * 8;
     if (lds > 80 * 1024) return false;
-    // the per-thread staging registers hold a row's window words and tap entries
+    // the per-thread staging registers hold a row's window words, phasor-table entries and rotations
     const int nthr = 64 * nwv, maxi = tpw == 1 ? 4 : 8;
-    if (Dk * (16 * JT + NWp) > maxi * nthr || Dk * NWp > TAB_MAX || CB * Dk * tap_pieces(Dk, NWp, nthr) > MAXG * nthr ||
+    if (Dk * (16 * JT + NWp) > maxi * nthr || Dk * NWp > TAB_MAX || CB * NE > ef_items(tpw) * nthr ||
         CB * (4 * JT + 1) > (tpw == 1 ? 1 : 2) * nthr)
         return false;
     // segments of output pulses: enough blocks to fill the device, a halo below a fifth of the rows
@@ -480,9 +488,8 @@ int launch_level_mfma(const float2* x, int Np, int P, int K, const float* c0, co
     if (G > 2147483647L) { set_error("level_mfma: too many blocks"); return E_INVALID; }
     const dim3 grid((unsigned)G), block((unsigned)(64 * nwv));
     const float kcf = (float)kc;
-    const int npc = tap_pieces(Dk, NWp, 64 * nwv);
 #define FSH_LM(A, B, T) hipLaunchKernelGGL((k_level_mfma<A, B, T>), grid, block, lds, s, x, P, K, c0, sl, C, hk, Lk, Dk, plk, \
-                                           Ko, NWp, NPH, PS, JT, njr, hp_t, Dp, plp, Lp, Po, IS, nseg, kcf, npc, z)
+                                           Ko, NWp, NPH, PS, JT, njr, hp_t, Dp, plp, Lp, Po, IS, nseg, kcf, z)
 #define FSH_LM_T(A, B)                                                                                                \
     switch (tpw) {                                                                                                    \
         case 1: FSH_LM(A, B, 1); break; case 2: FSH_LM(A, B, 2); break; default: FSH_LM(A, B, 3); break;             \

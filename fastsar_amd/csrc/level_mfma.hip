@@ -130,7 +130,8 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
     for (int i = tid; i < 8 * KS; i += nthr) {
         const int st = i >> 3, g = (i >> 1) & 3, pr = i & 1;
         const int col = 2 * st + pr, pb = col / (NWp >> 1), up = col - pb * (NWp >> 1);
-        taba[i] = (4 * pb + g) * PS + 2 * up;
+        // a padding column past the staged phase blocks reads plane g against its zero taps
+        taba[i] = 4 * pb < NPH ? (4 * pb + g) * PS + 2 * up : g * PS;
     }
     // the taps r = Dk u + ph < Lk in the order (ph, u), u fastest: entry i holds the byte offsets of its word in a
     // buffer's hi plane at lane 16 g (child 0, Re column) and of E[ph] and F[u] in a child's phasor table, and its
@@ -416,31 +417,41 @@ bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int
     if (Dk < 2 || Dp < 2 || C < 1 || Ko < 16) return false;
     const int NW = (Lk + Dk - 1) / Dk;
     NWp = NW + (NW & 1);
-    // phases padded to a multiple of 4 (one per lane group), NWp / 2 pairs each, two pair columns per k-step
-    const int cols = ((Dk + 3) / 4) * (NWp / 2);
+    // phases in blocks of 4 (one per lane group), NWp / 2 pairs each, two pair columns per k-step; a padding column
+    // (an odd count) reads staged planes against zero taps, so 4 ceil(Dk / 4) planes are staged
+    const int PB = (Dk + 3) / 4, cols = PB * (NWp / 2);
     KS = (cols + 1) / 2;
     if (KS < 1 || KS > 8) return false;
-    NPH = 4 * ((2 * KS + NWp / 2 - 1) / (NWp / 2));
+    NPH = 4 * PB;
     nwp = level_mfma_nw(Lp, Dp);
     if (nwp < 0) return false;
-    // output tiles: one per wave up to JT_MAX, else WAVES_MULTI waves with up to three each per block
+    const int NE = Dk + NW;
+    auto shape_lds = [&]() {
+        PS = 16 * JT + NWp;
+        PS = ((PS - 16 + 63) / 64) * 64 + 16;              // 16 mod 64 words: the lane groups' runs on disjoint banks
+        lds = (size_t)4 * 4 * NPH * PS + (size_t)2 * KS * 64 * 2 * 16 + (size_t)8 * KS * 4 + (size_t)TAB_MAX * 16 +
+              (size_t)2 * CB * (4 * JT + 1) * 8 + (size_t)2 * CB * NE * 8;
+    };
+    // output tiles: one per wave, a row of more than JT_MAX tiles cut into the fewest ranges (one block each) whose
+    // blocks stay within 80 KB of LDS (two a CU); FSH_LMFMA_MULTI=1 (A/B runs): such rows in blocks of WAVES_MULTI
+    // waves with up to three tiles each
     const int MT = (Ko + 15) / 16;
-    if (MT <= JT_MAX) {
-        njr = 1;
-        JT = MT;
-        tpw = 1;
-        nwv = JT;
-    } else {
+    const char* multi = getenv("FSH_LMFMA_MULTI");
+    if (MT > JT_MAX && multi && atoi(multi) == 1) {
         njr = (MT + 3 * WAVES_MULTI - 1) / (3 * WAVES_MULTI);
         JT = (MT + njr - 1) / njr;
         nwv = WAVES_MULTI;
         tpw = (JT + nwv - 1) / nwv;
+        shape_lds();
+    } else {
+        tpw = 1;
+        for (njr = (MT + JT_MAX - 1) / JT_MAX;; ++njr) {
+            JT = (MT + njr - 1) / njr;
+            shape_lds();
+            if (lds <= 80 * 1024 || JT == 1) break;
+        }
+        nwv = JT;
     }
-    PS = 16 * JT + NWp;
-    PS = ((PS - 16 + 63) / 64) * 64 + 16;                  // 16 mod 64 words: the lane groups' runs on disjoint banks
-    const int NE = Dk + NW;
-    lds = (size_t)4 * 4 * NPH * PS + (size_t)2 * KS * 64 * 2 * 16 + (size_t)8 * KS * 4 + (size_t)TAB_MAX * 16 +
-          (size_t)2 * CB * (4 * JT + 1) * 8 + (size_t)2 * CB * NE * 8;
     if (lds > 80 * 1024) return false;
     // the per-thread staging registers hold a row's window words, phasor-table entries and rotations
     const int nthr = 64 * nwv, maxi = tpw == 1 ? 4 : 8;
@@ -469,7 +480,8 @@ bool level_mfma_preferred(int C, int Lk, int Dk, int Ko, int Lp, int Dp, int Po)
     int KS, NWp, NPH, PS, JT, njr, nseg, IS, nwp, tpw, nwv;
     size_t lds;
     if (!level_mfma_shape(1, C, Lk, Dk, Ko, Lp, Dp, Po, KS, NWp, NPH, PS, JT, njr, nseg, IS, lds, nwp, tpw, nwv)) return false;
-    return (force && strcmp(force, "mfma") == 0) || (njr == 1 && tpw == 1);
+    const char* rows = getenv("FSH_LMFMA_ROWS");
+    return (force && strcmp(force, "mfma") == 0) || (njr == 1 && tpw == 1) || (rows && atoi(rows) == 1 && tpw == 1);
 }
 
 // hp_t: the pulse filter's polyphase table for level_mfma_nw(Lp, Dp) accumulators (launch_polyphase_nw)

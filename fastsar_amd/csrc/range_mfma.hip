@@ -210,16 +210,22 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
                 if (k < cnt) {
                     const int mt = mr + k;
                     floatx16_t acc = {};
-                    #pragma unroll
-                    for (int s = 0; s < KS; ++s) {
+                    // the A operand of k-step s + 1 read from LDS before k-step s's products issue (two in flight)
+                    auto load_a = [&](int s, uint4_t& wh, uint4_t& wl) {
                         const unsigned int* h0 = XH + off[s][0] + jr + 32 * k;
                         const unsigned int* h1 = XH + off[s][1] + jr + 32 * k;
                         const unsigned int* l0 = XL + off[s][0] + jr + 32 * k;
                         const unsigned int* l1 = XL + off[s][1] + jr + 32 * k;
-                        uint4_t wh, wl;
                         wh.x = h0[0]; wh.y = h0[1]; wh.z = h1[0]; wh.w = h1[1];
                         wl.x = l0[0]; wl.y = l0[1]; wl.z = l1[0]; wl.w = l1[1];
-                        const bf16x8_t Ah = __builtin_bit_cast(bf16x8_t, wh), Al = __builtin_bit_cast(bf16x8_t, wl);
+                    };
+                    uint4_t wh[2], wl[2];
+                    load_a(0, wh[0], wl[0]);
+                    #pragma unroll
+                    for (int s = 0; s < KS; ++s) {
+                        if (s + 1 < KS) load_a(s + 1, wh[(s + 1) & 1], wl[(s + 1) & 1]);
+                        __builtin_amdgcn_sched_barrier(0);     // the reads stay ahead of this k-step's products
+                        const bf16x8_t Ah = __builtin_bit_cast(bf16x8_t, wh[s & 1]), Al = __builtin_bit_cast(bf16x8_t, wl[s & 1]);
                         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gh[s], acc, 0, 0, 0);
                         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gl[s], acc, 0, 0, 0);
                         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Gh[s], acc, 0, 0, 0);

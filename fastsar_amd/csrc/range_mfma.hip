@@ -14,8 +14,11 @@
 // Work: block (row p, parent n, chunk of M-tiles), 256 threads. The row's window is staged once into LDS as packed
 // bfloat16 (Re, Im) words, hi and lo planes; wave w owns N-tiles (16 children) w, w + 4, ... (when there are fewer
 // than four, the waves share an N-tile and split its M-tiles), builds that N-tile's complex taps in registers (exact
-// float64 seed per pair of taps) and runs every M-tile of the chunk against them. Results are rotated by rho and
-// leave through a per-wave LDS transpose so that each half-wave stores 32 consecutive outputs of one child.
+// float64 seed per pair of taps) and runs every M-tile of the chunk against them. With rho applied here, results are
+// rotated and leave through a per-wave LDS transpose so that each half-wave stores 32 consecutive outputs of one
+// child; with rho deferred (ROT false) the product is formed transposed (the taps as the A operand), which leaves
+// each lane one output's (Re, Im) of eight children, stored straight from the accumulators, 32 consecutive outputs
+// of one child per half-wave.
 #include "fsh_common.h"
 #include "fsh_kernels.h"
 
@@ -226,17 +229,42 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
                         if (s + 1 < KS) load_a(s + 1, wh[(s + 1) & 1], wl[(s + 1) & 1]);
                         __builtin_amdgcn_sched_barrier(0);     // the reads stay ahead of this k-step's products
                         const bf16x8_t Ah = __builtin_bit_cast(bf16x8_t, wh[s & 1]), Al = __builtin_bit_cast(bf16x8_t, wl[s & 1]);
-                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gh[s], acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gl[s], acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Gh[s], acc, 0, 0, 0);
+                        if (ROT) {                     // D = Y [outputs x columns]
+                            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gh[s], acc, 0, 0, 0);
+                            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Gl[s], acc, 0, 0, 0);
+                            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Gh[s], acc, 0, 0, 0);
+                        } else {                       // D = Y^T [columns x outputs]: the taps as A, the window as B
+                            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Gh[s], Ah, acc, 0, 0, 0);
+                            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Gl[s], Ah, acc, 0, 0, 0);
+                            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Gh[s], Al, acc, 0, 0, 0);
+                        }
+                    }
+                    if (!ROT) {
+                        // Y^T: lane (output j = its column, half q) holds in acc[4 a + 2 e + {0, 1}] (Re, Im) of child
+                        // 16 nt + 4 a + 2 q + e, so each store of a half-wave writes 32 consecutive outputs of one child
+                        const int j = mt * 32 + col;
+                        const size_t o0 = (((size_t)n * Cn + nt * 16 + 2 * q) * P + p) * Ko + j, cs = (size_t)P * Ko;
+                        if (j < Ko && hs) {            // float16 storage, scaled by hs[0] (a power of two)
+                            typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+                            const float gs = hs[0];
+                            #pragma unroll
+                            for (int i = 0; i < 8; ++i) {
+                                const int d = 4 * (i >> 1) + (i & 1);      // child 16 nt + 2 q + d
+                                if (nt * 16 + 2 * q + d < Cn)
+                                    reinterpret_cast<h2*>(y)[o0 + d * cs] = h2{(_Float16)(acc[2 * i] * gs), (_Float16)(acc[2 * i + 1] * gs)};
+                            }
+                        } else if (j < Ko) {
+                            #pragma unroll
+                            for (int i = 0; i < 8; ++i) {
+                                const int d = 4 * (i >> 1) + (i & 1);
+                                if (nt * 16 + 2 * q + d < Cn) y[o0 + d * cs] = make_float2(acc[2 * i], acc[2 * i + 1]);
+                            }
+                        }
+                        continue;
                     }
                     // rho at rows 4q, 4q + 1, ... of this tile: seeded at row 4q, stepped by D samples, 8 rows apart
                     // by 8 D; the lane pair (t = 0, 1) holds (Re, Im) of one output: Re' = Re er - Im ei,
                     // Im' = Im er + Re ei
-                    if (!ROT) {
-                        #pragma unroll
-                        for (int i = 0; i < 16; ++i) stg[(col >> 1) * 66 + 2 * (8 * (i >> 2) + 4 * q + (i & 3)) + t] = acc[i];
-                    }
                     float rr = 1.f, ri = 0.f;
                     if (ROT) cis_cyc(fma((double)(D * (mt * 32 + 4 * q) - pl) - kc, b, a), rr, ri);
                     #pragma unroll

@@ -425,7 +425,7 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
                                                        const float* __restrict__ taps, int L, int pl, int D, int Po, int IC,
                                                        float2* __restrict__ z, const float* __restrict__ hs,
                                                        const float* __restrict__ rc0, const float* __restrict__ rsl, int rD,
-                                                       int rpl, double rkc)
+                                                       int rpl, double rkc, int Ys)
 {
     typedef _Float16 h2 __attribute__((ext_vector_type(2)));
     const float hinv = hs ? hs[1] : 1.f;             // float16 input: its scale undone by hs[1]
@@ -439,8 +439,8 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
     if (col >= (long)B * Ko) return;
     const int b = (int)(col / Ko), j = (int)(col - (long)b * Ko);
     const int i0 = blockIdx.y * IC, i1 = min(Po, i0 + IC);
-    const float2* yb = y + (size_t)b * P * Ko + j;
-    const h2* ybh = yh + (size_t)b * P * Ko + j;
+    const float2* yb = y + (size_t)b * P * Ys + j;           // rows of y are Ys samples apart
+    const h2* ybh = yh + (size_t)b * P * Ys + j;
     float2* zb = z + (size_t)b * Po * Ko + j;
     // a deferred range rotation: sample (p, j) of row b times exp(2 pi i (c0 + (rD j - rpl - rkc) sl)) of row p; the
     // offset rD j - rpl - rkc is exact in float32 (an integer or a half below 2^23)
@@ -466,10 +466,10 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
                     rb[u] = ra[u] = 0.f;
                     if (ph0 + u < D && p >= 0 && p < P) {
                         if (hs) {
-                            const h2 w = ybh[(size_t)p * Ko];
+                            const h2 w = ybh[(size_t)p * Ys];
                             v[u] = make_float2((float)w.x * hinv, (float)w.y * hinv);
                         } else {
-                            v[u] = yb[(size_t)p * Ko];
+                            v[u] = yb[(size_t)p * Ys];
                         }
                         if (rc0b) { rb[u] = rslb[p]; ra[u] = rc0b[p]; }
                     }
@@ -503,10 +503,10 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
                 for (int u = 0; u < 4; ++u) {
                     const int p = pbase + ph0 + u;
                     if (hs) {
-                        const h2 w = ybh[(size_t)min(max(p, 0), P - 1) * Ko];
+                        const h2 w = ybh[(size_t)min(max(p, 0), P - 1) * Ys];
                         v[u] = make_float2((float)w.x * hinv, (float)w.y * hinv);
                     } else {
-                        v[u] = yb[(size_t)min(max(p, 0), P - 1) * Ko];
+                        v[u] = yb[(size_t)min(max(p, 0), P - 1) * Ys];
                     }
                 }
                 #pragma unroll
@@ -532,15 +532,16 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
     }
 }
 
-// The streaming pulse filter with two columns per thread, j and j + ceil(Ko / 2) of one row b: twice the row loads
+// The streaming pulse filter with two columns per thread, j and j + ceil(Ys / 2) of one row b: twice the row loads
 // in flight per wave and two independent accumulator windows, the rows of a period loaded four at a time. The same
-// sums, rotation and output as k_fir_p_stream.
+// sums, rotation and output as k_fir_p_stream. Rows of y lie Ys >= Ko samples apart: with Ys a multiple of 32 every
+// wave's loads start on a 128-byte line (columns past Ko are read and not stored).
 template <int NW>
 __global__ __launch_bounds__(256) void k_fir_p_stream2(const float2* __restrict__ y, int B, int P, int Ko,
                                                         const float* __restrict__ taps, int L, int pl, int D, int Po, int IC,
                                                         float2* __restrict__ z, const float* __restrict__ hs,
                                                         const float* __restrict__ rc0, const float* __restrict__ rsl, int rD,
-                                                        int rpl, double rkc)
+                                                        int rpl, double rkc, int Ys)
 {
     typedef _Float16 h2 __attribute__((ext_vector_type(2)));
     constexpr int RB = 4;
@@ -548,14 +549,14 @@ __global__ __launch_bounds__(256) void k_fir_p_stream2(const float2* __restrict_
     extern __shared__ float tpz[];                   // [D * NW], zero beyond L
     for (int r = threadIdx.x; r < D * NW; r += 256) tpz[r] = r < L ? taps[r] : 0.f;
     __syncthreads();
-    const int H = (Ko + 1) / 2;
+    const int H = (Ys + 1) / 2;
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= (long)B * H) return;
     const int b = (int)(t / H), ja = (int)(t - (long)b * H), jb = ja + H;
-    const bool two = jb < Ko;
-    const int jb0 = two ? jb : ja;                   // the second column's loads stay inside the row when it is absent
+    const bool one = ja < Ko, two = jb < Ko;
+    const int jb0 = jb < Ys ? jb : ja;               // the second column's loads stay inside the row when it is absent
     const int i0 = blockIdx.y * IC, i1 = min(Po, i0 + IC);
-    const size_t base = (size_t)b * P * Ko;
+    const size_t base = (size_t)b * P * Ys;
     const float2* ya = y + base + ja;
     const float2* yb2 = y + base + jb0;
     const h2* yha = reinterpret_cast<const h2*>(y) + base + ja;
@@ -586,12 +587,12 @@ __global__ __launch_bounds__(256) void k_fir_p_stream2(const float2* __restrict_
                 rb[u] = ra[u] = 0.f;
                 if (ph0 + u < D && p >= 0 && p < P) {
                     if (hs) {
-                        const h2 wa = yha[(size_t)p * Ko], wb = yhb[(size_t)p * Ko];
+                        const h2 wa = yha[(size_t)p * Ys], wb = yhb[(size_t)p * Ys];
                         va[u] = make_float2((float)wa.x * hinv, (float)wa.y * hinv);
                         vb[u] = make_float2((float)wb.x * hinv, (float)wb.y * hinv);
                     } else {
-                        va[u] = ya[(size_t)p * Ko];
-                        vb[u] = yb2[(size_t)p * Ko];
+                        va[u] = ya[(size_t)p * Ys];
+                        vb[u] = yb2[(size_t)p * Ys];
                     }
                     if (rc0b) { rb[u] = rslb[p]; ra[u] = rc0b[p]; }
                 }
@@ -617,7 +618,7 @@ __global__ __launch_bounds__(256) void k_fir_p_stream2(const float2* __restrict_
         }
         const int done = m - NW + 1;
         if (done >= i0 && done < i1) {
-            za[(size_t)done * Ko] = acca[0];
+            if (one) za[(size_t)done * Ko] = acca[0];
             if (two) zb2[(size_t)done * Ko] = accb[0];
         }
         #pragma unroll
@@ -1071,7 +1072,7 @@ int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, c
 }
 
 int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
-                     int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s, bool* deferred)
+                     int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s, bool* deferred, int ys)
 {
     if (deferred) *deferred = false;
     if (Np < 1 || P < 1 || K < 1 || Cn < 1 || L < 1 || D < 1 || Ko < 1 || pl < 0) {
@@ -1091,7 +1092,8 @@ int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, con
         // the row form amortizes each row's taps over its M-tiles: it wins from four M-tiles (128 outputs) up
         const bool row_form = force && strcmp(force, "mfma") == 0 ? true : (Ko + 31) / 32 >= 4;
         if (!want_mfma1 && row_form) {
-            st = launch_rot_fir_k_mf2(x, Np, P, K, c0, sl, Cn, taps, L, pl, D, Ko, kc, y, s, nullptr, deferred == nullptr);
+            st = launch_rot_fir_k_mf2(x, Np, P, K, c0, sl, Cn, taps, L, pl, D, Ko, kc, y, s, nullptr, deferred == nullptr,
+                                      deferred ? ys : 0);
             if (st == OK && deferred) *deferred = true;
         }
         if (st == E_INVALID) st = launch_rot_fir_k_mfma(x, Np, P, K, c0, sl, Cn, taps, L, pl, D, Ko, kc, y, s);
@@ -1133,9 +1135,10 @@ bool fir_p_streams(int L, int D)
 }
 
 int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float2* z, hipStream_t s,
-                 const float* hs, const DeferredRot* rot)
+                 const float* hs, const DeferredRot* rot, int ys)
 {
-    if (B < 1 || P < 1 || Ko < 1 || L < 1 || D < 1 || Po < 1 || pl < 0) {
+    const int Ys = ys > 0 ? ys : Ko;
+    if (B < 1 || P < 1 || Ko < 1 || L < 1 || D < 1 || Po < 1 || pl < 0 || Ys < Ko) {
         set_error("fir_p: invalid shape");
         return E_INVALID;
     }
@@ -1145,8 +1148,8 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
         set_error("fir_p: a float16 input is read by the streaming kernel only (at most 16 taps per phase)");
         return E_INVALID;
     }
-    if (rot && !fir_p_streams(L, D)) {
-        set_error("fir_p: a deferred range rotation is applied by the streaming kernel only");
+    if ((rot || Ys != Ko) && !fir_p_streams(L, D)) {
+        set_error("fir_p: a deferred range rotation or a padded row stride is taken by the streaming kernel only");
         return E_INVALID;
     }
     if (NW <= 16 && !(force && strcmp(force, "lds") == 0)) {
@@ -1164,10 +1167,10 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
         const char* cle = getenv("FSH_FIRP_COLS");
         const int cols = cle ? atoi(cle) : ((long)B * Ko >= (1L << 17) ? 2 : 1);
         if (!batch && cols == 2) {
-            const int H = (Ko + 1) / 2;
+            const int H = (Ys + 1) / 2;
             dim3 grid2((unsigned)(((long)B * H + 255) / 256), (Po + IC - 1) / IC);
 #define FSH_FP2(NN) hipLaunchKernelGGL((k_fir_p_stream2<NN>), grid2, dim3(256), (size_t)4 * D * NW, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs, \
-                                       rc0, rsl, rD, rpl, rkc)
+                                       rc0, rsl, rD, rpl, rkc, Ys)
             switch (NW) {
                 case 1: FSH_FP2(1); break;   case 2: FSH_FP2(2); break;   case 3: FSH_FP2(3); break;   case 4: FSH_FP2(4); break;
                 case 5: FSH_FP2(5); break;   case 6: FSH_FP2(6); break;   case 7: FSH_FP2(7); break;   case 8: FSH_FP2(8); break;
@@ -1182,13 +1185,13 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
 #define FSH_FP(NN)                                                                                                       \
     do {                                                                                                                 \
         if (batch) hipLaunchKernelGGL((k_fir_p_stream<NN, true, 1>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, \
-                                      hs, rc0, rsl, rD, rpl, rkc);                                                       \
+                                      hs, rc0, rsl, rD, rpl, rkc, Ys);                                                   \
         else if (rows == 8) hipLaunchKernelGGL((k_fir_p_stream<NN, false, 8>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, \
-                                               Po, IC, z, hs, rc0, rsl, rD, rpl, rkc);                                   \
+                                               Po, IC, z, hs, rc0, rsl, rD, rpl, rkc, Ys);                               \
         else if (rows == 4) hipLaunchKernelGGL((k_fir_p_stream<NN, false, 4>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, \
-                                               Po, IC, z, hs, rc0, rsl, rD, rpl, rkc);                                   \
+                                               Po, IC, z, hs, rc0, rsl, rD, rpl, rkc, Ys);                               \
         else hipLaunchKernelGGL((k_fir_p_stream<NN, false, 1>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs, \
-                                rc0, rsl, rD, rpl, rkc);                                                                 \
+                                rc0, rsl, rD, rpl, rkc, Ys);                                                             \
     } while (0)
         switch (NW) {
             case 1: FSH_FP(1); break;

@@ -241,7 +241,7 @@ struct Former {
             const long long ch = (l == 0) ? g : (long long)h.C;
             const long long outs = parents * ch;
             const bool one_pass = lv[l].fused || lv[l].mfma;
-            if (!one_pass) maxbuf = std::max(maxbuf, (size_t)(outs * h.P * h.Ko));
+            if (!one_pass) maxbuf = std::max(maxbuf, (size_t)(outs * h.P * padded_row(h.Ko)));   // y, rows padded
             if (h.Dp > 1 || one_pass) maxbuf = std::max(maxbuf, (size_t)(outs * h.Po * h.Ko));
             if (l >= 2) maxph = std::max(maxph, (size_t)(parents * h.C * h.P));
             parents = outs;
@@ -369,20 +369,27 @@ struct Former {
             const char* dr_env = getenv("FSH_DEFER_ROT");
             const bool can_defer = h.Dp > 1 && fir_p_streams(h.Lp, h.Dp) && !(dr_env && strcmp(dr_env, "0") == 0);
             bool deferred = false;
+            // the row form writes y's rows padded to whole 128-byte lines for the streaming pulse filter (ystr)
+            const int Kp = padded_row(h.Ko);
+            int ystr = h.Ko;
             if (half_mid && h.Dk > 1 && h.Dp > 1 && (h.Lp + h.Dp - 1) / h.Dp <= 16 && !getenv("FSH_FIRP_KERNEL")) {
                 st = launch_rot_fir_k_mf2(cur, (int)parents, h.P, h.K, c0, sl, ch, L.taps_k, h.Lk, h.plk, h.Dk, h.Ko, h.kc,
-                                          buf[yb], stream, hsc + 2 * l, !can_defer);
+                                          buf[yb], stream, hsc + 2 * l, !can_defer, Kp);
                 if (st == OK) {
                     hs = hsc + 2 * l;
                     deferred = can_defer;
+                    ystr = Kp;
                 } else if (st != E_INVALID) {
                     return st;
                 }
             }
-            if (!hs && (st = launch_rot_fir_k(cur, (int)parents, h.P, h.K, c0, sl, ch, h.Dk > 1 ? L.taps_k : one,
-                                              h.Dk > 1 ? h.Lk : 1, h.Dk > 1 ? h.plk : 0, h.Dk, h.Ko, h.kc, buf[yb], stream,
-                                              can_defer ? &deferred : nullptr)))
-                return st;
+            if (!hs) {
+                if ((st = launch_rot_fir_k(cur, (int)parents, h.P, h.K, c0, sl, ch, h.Dk > 1 ? L.taps_k : one,
+                                           h.Dk > 1 ? h.Lk : 1, h.Dk > 1 ? h.plk : 0, h.Dk, h.Ko, h.kc, buf[yb], stream,
+                                           can_defer ? &deferred : nullptr, Kp)))
+                    return st;
+                if (deferred) ystr = Kp;
+            }
             const DeferredRot drot{c0, sl, h.Dk, h.Dk > 1 ? h.plk : 0, h.kc};
             mark_end(ST_ROT);
             cur = buf[yb];
@@ -390,7 +397,7 @@ struct Former {
             if (h.Dp > 1) {
                 mark_begin();
                 if ((st = launch_fir_p(buf[yb], (int)(parents * ch), h.P, h.Ko, L.taps_p, h.Lp, h.plp, h.Dp, h.Po, buf[1 - yb],
-                                       stream, hs, deferred ? &drot : nullptr)))
+                                       stream, hs, deferred ? &drot : nullptr, ystr)))
                     return st;
                 mark_end(ST_FIRP);
                 cur = buf[1 - yb];

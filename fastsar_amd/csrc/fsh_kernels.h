@@ -17,14 +17,18 @@ int launch_prepare16(const void* S, int dtype, const float* inv, int P, int K, c
                      int* bad, hipStream_t s, unsigned* xmax = nullptr, const double* ramp = nullptr, int p0 = 0);
 int launch_fill_int(int* p, int n, int value, hipStream_t s);
 // deferred != nullptr: the caller's pulse filter can apply rho (a launch_fir_p with the same c0, sl, D, pl, kc), and
-// *deferred tells whether the chosen kernel left it out of y
+// *deferred tells whether the chosen kernel left it out of y; that kernel (the row form) then writes rows ys samples
+// apart (ys >= Ko; 0: Ko), the others always Ko apart
 int launch_rot_fir_k(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
-                     int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s, bool* deferred = nullptr);
+                     int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s, bool* deferred = nullptr, int ys = 0);
+// the row stride of a two-pass level's range-filtered rows between the row form and the streaming pulse filter: Ko
+// rounded up to 32 samples (whole 128-byte lines for float2 and float16 pairs) from 128 samples up
+inline int padded_row(int Ko) { return Ko >= 128 ? (Ko + 31) / 32 * 32 : Ko; }
 // the row form of the matrix-core range filter (range_mfma.hip); E_INVALID when the taps do not fit it. hs: store
 // the output as float16 pairs times hs[0] (4 bytes a sample) instead of float2
 int launch_rot_fir_k_mf2(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
                          int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s, const float* hs = nullptr,
-                         bool rot = true);
+                         bool rot = true, int ys = 0);
 // the range stage's rho_b(p, j) = exp(2 pi i (c0[b P + p] + (D j - pl - kc) sl[b P + p])), left for the pulse filter
 struct DeferredRot {
     const float* c0;
@@ -32,10 +36,10 @@ struct DeferredRot {
     int D, pl;
     double kc;
 };
-// hs: the input is float16 pairs, times hs[1] when read; rot: each input sample times rho first (both: the streaming
-// kernel only)
+// hs: the input is float16 pairs, times hs[1] when read; rot: each input sample times rho first; ys: rows of y ys >= Ko
+// samples apart (0: Ko) (all three: the streaming kernel only)
 int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float2* z,
-                 hipStream_t s, const float* hs = nullptr, const DeferredRot* rot = nullptr);
+                 hipStream_t s, const float* hs = nullptr, const DeferredRot* rot = nullptr, int ys = 0);
 // whether launch_fir_p takes the streaming kernel for this filter (the one that applies a deferred rho)
 bool fir_p_streams(int L, int D);
 // the rotation and both filters of a level in one pass, for short children (fused_kernels.hip); hk_t and hp_t are the

@@ -62,7 +62,7 @@ template <int KS, bool ROT>
 __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restrict__ x, int P, int K, const float* __restrict__ c0,
                                                         const float* __restrict__ sl, int Cn, const float* __restrict__ taps,
                                                         int L, int pl, int D, int Ko, double kc, int NWp, int NPH, int PS,
-                                                        int MC, float2* __restrict__ y, const float* __restrict__ hs)
+                                                        int MC, float2* __restrict__ y, const float* __restrict__ hs, int Ys)
 {
     extern __shared__ unsigned int lds_u[];
     unsigned int* XH = lds_u;                                  // [NPH][PS]
@@ -201,9 +201,9 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
             cis_cyc(b * (double)(8 * D), s8r, s8i);
         }
         // outputs: child nt 16 + q + 2 it at row p, sample j, as base + it (2 P Ko) + lane offset (q P Ko + j)
-        const size_t ybase = (((size_t)n * Cn + nt * 16) * P + p) * Ko;
-        const size_t ystep = 2 * (size_t)P * Ko;
-        const unsigned int ylane = (unsigned int)q * (unsigned int)P * (unsigned int)Ko + (unsigned int)col;
+        const size_t ybase = (((size_t)n * Cn + nt * 16) * P + p) * Ys;
+        const size_t ystep = 2 * (size_t)P * Ys;
+        const unsigned int ylane = (unsigned int)q * (unsigned int)P * (unsigned int)Ys + (unsigned int)col;
         // runs of up to 8 M-tiles: tile k of a run reads 32 k words past the run's first, an immediate offset
         for (int mr = ma; mr < mb; mr += 8) {
             const int cnt = min(8, mb - mr);
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
                         // Y^T: lane (output j = its column, half q) holds in acc[4 a + 2 e + {0, 1}] (Re, Im) of child
                         // 16 nt + 4 a + 2 q + e, so each store of a half-wave writes 32 consecutive outputs of one child
                         const int j = mt * 32 + col;
-                        const size_t o0 = (((size_t)n * Cn + nt * 16 + 2 * q) * P + p) * Ko + j, cs = (size_t)P * Ko;
+                        const size_t o0 = (((size_t)n * Cn + nt * 16 + 2 * q) * P + p) * Ys + j, cs = (size_t)P * Ys;
                         if (j < Ko && hs) {            // float16 storage, scaled by hs[0] (a power of two)
                             typedef _Float16 h2 __attribute__((ext_vector_type(2)));
                             const float gs = hs[0];
@@ -341,12 +341,14 @@ static bool mf2_shape(int L, int D, int Ko, int& NWp, int& KS, int& NPH, int& PS
 }
 
 int launch_rot_fir_k_mf2(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int Cn, const float* taps,
-                         int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s, const float* hs, bool rot)
+                         int L, int pl, int D, int Ko, double kc, float2* y, hipStream_t s, const float* hs, bool rot, int ys)
 {
+    const int Ys = ys > 0 ? ys : Ko;                            // row stride of y (samples)
+    if (Ys < Ko) return E_INVALID;
     int NWp, KS, NPH, PS, MC, chunks;
     size_t lds;
     if (!mf2_shape(L, D, Ko, NWp, KS, NPH, PS, MC, chunks, lds)) return E_INVALID;
-    if ((size_t)P * Ko >= ((size_t)1 << 31)) {                // the stores' lane offsets are 32-bit
+    if ((size_t)P * Ys >= ((size_t)1 << 31)) {                // the stores' lane offsets are 32-bit
         set_error("rot_fir_k_mf2: rows of more than 2^31 outputs a child");
         return E_INVALID;
     }
@@ -357,14 +359,14 @@ int launch_rot_fir_k_mf2(const float2* x, int Np, int P, int K, const float* c0,
         const float* c0s = c0 + (size_t)n0 * Cn * P;
         const float* sls = sl + (size_t)n0 * Cn * P;
         // a float16 output counts 4 bytes a sample: the chunk's offset in float2 units is halved
-        float2* ys = hs ? reinterpret_cast<float2*>(reinterpret_cast<char*>(y) + (size_t)n0 * Cn * P * Ko * 4)
-                        : y + (size_t)n0 * Cn * P * Ko;
+        float2* yc = hs ? reinterpret_cast<float2*>(reinterpret_cast<char*>(y) + (size_t)n0 * Cn * P * Ys * 4)
+                        : y + (size_t)n0 * Cn * P * Ys;
 #define FSH_M2(KK)                                                                                                      \
     do {                                                                                                                \
         if (rot) hipLaunchKernelGGL((k_rot_fir_k_mf2<KK, true>), grid, dim3(256), lds, s, xs, P, K, c0s, sls, Cn, taps, L, pl, \
-                                    D, Ko, kc, NWp, NPH, PS, MC, ys, hs);                                               \
+                                    D, Ko, kc, NWp, NPH, PS, MC, yc, hs, Ys);                                               \
         else hipLaunchKernelGGL((k_rot_fir_k_mf2<KK, false>), grid, dim3(256), lds, s, xs, P, K, c0s, sls, Cn, taps, L, pl, \
-                                D, Ko, kc, NWp, NPH, PS, MC, ys, hs);                                                   \
+                                D, Ko, kc, NWp, NPH, PS, MC, yc, hs, Ys);                                                   \
     } while (0)
         switch (KS) {
             case 1: FSH_M2(1); break;   case 2: FSH_M2(2); break;   case 3: FSH_M2(3); break;   case 4: FSH_M2(4); break;

@@ -53,6 +53,12 @@ GROUP = 8                  # first-level children per group at full speed
 GROUP_BUDGET = 4 << 30     # groups grow past GROUP, up to every first-level child, while their buffers stay within this
 
 
+def padded_row(Ko):
+    """Row stride (samples) of a two-pass level's range-filtered rows on the device: Ko rounded up to 32 from 128
+    samples up, so that each row starts on a 128-byte line."""
+    return (Ko + 31) // 32 * 32 if Ko >= 128 else Ko
+
+
 def hip_group_bytes(plan, g, forms):
     """Bytes of the buffers of a group of g first-level children: two ping-pong buffers of the largest level output
     (and, for a two-pass level, its range-filtered intermediate) in complex64, the device-computed rotations of the
@@ -62,8 +68,8 @@ def hip_group_bytes(plan, g, forms):
     for l, lv in enumerate(plan['levels']):
         outs = parents * (g if l == 0 else lv['C'])
         one = forms[l] != 0
-        if not one:
-            maxbuf = max(maxbuf, outs * lv['P'] * lv['Ko'])
+        if not one:        # the range-filtered intermediate, its rows padded as the device pads them
+            maxbuf = max(maxbuf, outs * lv['P'] * padded_row(lv['Ko']))
         if lv['Dp'] > 1 or one:
             maxbuf = max(maxbuf, outs * lv['Po'] * lv['Ko'])
         if l >= 2:

@@ -372,6 +372,7 @@ def main():
     ap.add_argument('--batch-atten', type=float, default=80.0, help='stopband attenuation of the batch formers\' filters (dB)')
     ap.add_argument('--taps', default=None, help='learned decimation filters (an .npz of fastsar_amd.learned) for the image')
     ap.add_argument('--no-learned', action='store_true', help='skip the learned form beside the plain one')
+    ap.add_argument('--no-valu-form', action='store_true', help='skip the float32 VALU form beside the matrix-core one')
     ap.add_argument('--train-taps', type=int, default=0, help='train filters of this many taps per phase on 2 scenes of '
                     'the workload first (seeds 11, 12), saved to --taps-out, and form with them')
     ap.add_argument('--taps-out', default=None, help='where --train-taps saves its filters (default learned_T<n>.npz)')
@@ -595,6 +596,39 @@ def main():
         log(f'learned form: {learned_form}')
         del limg
 
+    # the same image with every stage on float32 VALU kernels (no matrix cores, no bfloat16 operands): the plain float32
+    # form that the matrix-core form (bfloat16 x 3 operands, float32 accumulation) and the learned float16 form stand
+    # beside, timed on one warm and two timed calls
+    valu_form = None
+    if not args.small and not args.no_valu_form and taps is None and args.precision == 'float32' and args.history == 'complex64':
+        keep = {k: os.environ.get(k) for k in ('FSH_ROT_KERNEL', 'FSH_LEVEL_KERNEL')}
+        os.environ['FSH_ROT_KERNEL'], os.environ['FSH_LEVEL_KERNEL'] = 'valu', 'split'
+        try:
+            vimg = _hip.DeviceArray((grid['nx'], grid['ny']), np.complex64)
+            vf = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=args.T, levels=args.levels, atten=args.atten)
+            vf(hist, out=vimg)
+            _hip.check(_hip.lib().fsh_synchronize())
+            t1 = time.perf_counter()
+            for _ in range(2):
+                vf(hist, out=vimg)
+            _hip.check(_hip.lib().fsh_synchronize())
+            vdt = (time.perf_counter() - t1) / 2
+            vh = vimg.to_host()
+            verr = rel_db(np.concatenate([vh[i0:i0 + side, j0:j0 + side].ravel() for i0, j0 in tiles]), ref_vals)
+            del vf, vimg, vh
+        finally:
+            for k, v in keep.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
+        valu_form = dict(ms_per_step=round(vdt * 1e3, 2), value=round(pairs / vdt / 1e9, 2), error_db=round(verr, 2),
+                         matrix_core_speedup=round(vdt / dt, 3),
+                         learned_float16_speedup=(round(vdt / (learned_form['float16']['ms_per_step'] * 1e-3), 3)
+                                                  if learned_form else None),
+                         kernels='rot_fir_k VALU (float32 taps per row), fir_p streaming, level_fused VALU, final stage VALU')
+        log(f'float32 VALU form: {valu_form}')
+
     exact = None
     if not args.no_exact and not args.small and args.history == 'complex64':
         exact = exact_leg(col, grid, dS, tiles, side, ref_vals, args.steps)
@@ -630,6 +664,7 @@ def main():
                            predicted_error_db=round(form_pred, 2), parallelism='single GPU',
                            host_to_host_ms=None if h2h is None else round(h2h * 1e3, 1),
                            host_to_host_steps=args.host_steps if h2h is not None else 0, learned_form=learned_form,
+                           fp32_valu_form=valu_form,
                            exact_form=exact, batch_form=batch),
                roofline=roofline, cpu_baseline=cpu, stages=stages)
     print(json.dumps(out), flush=True)

@@ -73,6 +73,27 @@ def test_full_image_error_and_groups(full):
 
 
 @pytest.mark.timeout(600)
+def test_full_image_float16_rows(full):
+    """precision='float16' at full size: level 1's range-filtered rows stored as scaled float16 pairs, 320 samples
+    apart (rows of 315 padded to whole 128-byte lines), the image within 0.5 dB of the float32 one against the float64
+    checker on the same tiles"""
+    col, grid, pos, dS = full
+    rng = np.random.default_rng(11)
+    tiles = [(int(rng.integers(0, grid['nx'] - 255)), int(rng.integers(0, grid['ny'] - 255))) for _ in range(4)]
+    ref = _checker(dS, col, _tile_points(grid, tiles, 256))
+    img = _hip.DeviceArray((grid['nx'], grid['ny']), np.complex64)
+    errs = {}
+    for prec in ('float32', 'float16'):
+        f = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=8, levels=4, atten=80.0, precision=prec)
+        f(dS, out=img)
+        a = img.to_host()
+        errs[prec] = rel_db(np.concatenate([a[i:i + 256, j:j + 256].ravel() for i, j in tiles]), ref)
+        del f
+    print(f'  float32 {errs["float32"]:.2f} dB, float16 rows {errs["float16"]:.2f} dB on 4 tiles of 256^2', flush=True)
+    assert errs['float32'] <= -60.0 and errs['float16'] <= errs['float32'] + 0.5
+
+
+@pytest.mark.timeout(600)
 def test_full_geometry_checker_against_direct_sum(full):
     """the float64 device checker (64x oversampled complex128 profiles, cubic reads) against the interpolation-free
     float64 sum over all 16384 x 8192 samples, on the CPU, at four target pixels and four seeded ones"""

@@ -59,9 +59,13 @@ __global__ __launch_bounds__(256) void k_prepare(const float2* __restrict__ S, i
 
 // The rotation folded into the taps: with k_j = D j - pl,
 //   y_j = sum_r taps[r] x[k_j + r] e^{2 pi i (c0 + (k_j + r - kc) sl)} = e^{2 pi i (c0 + (k_j - kc) sl)} sum_r h_r x[k_j + r],
-// h_r = taps[r] e^{2 pi i r sl}: per (row, child) L complex taps formed in float64 and one phasor per output, so no
-// rotated copy of the window is made. grid (ceil(Ko / JB), ceil(P / PB), Np), block 256. Dynamic LDS: the window of
-// the block's rows [PB][W] (float2, W = D (JB - 1) + L, loaded once for every child) and the complex taps [PB][L].
+// h_r = taps[r] e^{2 pi i r sl}: per (row, child) L complex taps and one phasor per output, so no rotated copy of the
+// window is made. A thread forms a run of consecutive taps of one row (one tap where the block has a thread per tap):
+// the run's first ramp value seeded in float64, then stepped by a float32 recurrence; the output phasor in float64.
+// grid (ceil(Ko / JB), ceil(P /
+// PB), Np), block 256; where the block has fewer outputs than threads, SO = 256 / (PB JB) consecutive threads share
+// an output, each summing every SO-th tap, added across them in a fixed order. Dynamic LDS: the window of the
+// block's rows [PB][W] (float2, W = D (JB - 1) + L, loaded once for every child) and the complex taps [PB][L].
 __global__ __launch_bounds__(256) void k_rot_fir_k(const float2* __restrict__ x, int P, int K, const float* __restrict__ c0,
                                                     const float* __restrict__ sl, int Cn, const float* __restrict__ taps, int L,
                                                     int pl, int D, int Ko, double kc, int JB, int PB, float2* __restrict__ y)
@@ -81,27 +85,49 @@ __global__ __launch_bounds__(256) void k_rot_fir_k(const float2* __restrict__ x,
     for (int c = 0; c < Cn; ++c) {
         const size_t rc = ((size_t)n * Cn + c) * P;
         __syncthreads();                                   // the window is in; the previous child's taps are read
-        for (int e = tid; e < PB * L; e += 256) {
-            const int rr = e / L, r = e - rr * L, p = min(p0 + rr, P - 1);
-            float cs, sn;
-            cis_cyc((double)r * (double)sl[rc + p], cs, sn);
-            h[e] = make_float2(taps[r] * cs, taps[r] * sn);
+        // runs of RL consecutive taps of one row a thread (RL = 1 while every tap has a thread of its own): the
+        // run's first ramp value and its step seeded in float64, the rest by a float32 recurrence
+        int RL = 1;
+        while (PB * ((L + RL - 1) / RL) > 256) ++RL;
+        const int NR = (L + RL - 1) / RL;
+        for (int e = tid; e < PB * NR; e += 256) {
+            const int rr = e / NR, r0 = (e - rr * NR) * RL, r1 = min(L, r0 + RL), p = min(p0 + rr, P - 1);
+            const double b = (double)sl[rc + p];
+            float cs, sn, dc = 1.f, dn = 0.f;
+            cis_cyc((double)r0 * b, cs, sn);
+            if (r1 - r0 > 1) cis_cyc(b, dc, dn);
+            for (int r = r0; r < r1; ++r) {
+                h[(size_t)rr * L + r] = make_float2(taps[r] * cs, taps[r] * sn);
+                const float nc = fmaf(cs, dc, -(sn * dn));
+                sn = fmaf(cs, dn, sn * dc);
+                cs = nc;
+            }
         }
         __syncthreads();
-        for (int o = tid; o < PB * JB; o += 256) {
+        const int SO = PB * JB >= 256 ? 1 : 256 / (PB * JB);     // threads per output (a power of two up to 64)
+        for (int t0 = tid; t0 < PB * JB * SO; t0 += 256) {
+            const int o = t0 / SO, so = t0 - o * SO;
             const int rr = o / JB, jj = o - rr * JB, p = p0 + rr, j = j0 + jj;
-            if (p >= P || j >= Ko) continue;
+            const bool ok = p < P && j < Ko;
             const float2* xs = win + (size_t)rr * W + D * jj;
             const float2* hr = h + (size_t)rr * L;
             float ar = 0.f, ai = 0.f;
-            for (int r = 0; r < L; ++r) {
-                const float2 v = xs[r], t = hr[r];
-                ar = fmaf(t.x, v.x, fmaf(-t.y, v.y, ar));
-                ai = fmaf(t.x, v.y, fmaf(t.y, v.x, ai));
+            if (ok) {
+                for (int r = so; r < L; r += SO) {
+                    const float2 v = xs[r], t = hr[r];
+                    ar = fmaf(t.x, v.x, fmaf(-t.y, v.y, ar));
+                    ai = fmaf(t.x, v.y, fmaf(t.y, v.x, ai));
+                }
             }
-            float cs, sn;
-            cis_cyc(fma((double)(D * j - pl) - kc, (double)sl[rc + p], (double)c0[rc + p]), cs, sn);
-            y[(rc + p) * Ko + j] = make_float2(fmaf(ar, cs, -(ai * sn)), fmaf(ar, sn, ai * cs));
+            for (int m = 1; m < SO; m <<= 1) {
+                ar += __shfl_xor(ar, m, 64);
+                ai += __shfl_xor(ai, m, 64);
+            }
+            if (ok && so == 0) {
+                float cs, sn;
+                cis_cyc(fma((double)(D * j - pl) - kc, (double)sl[rc + p], (double)c0[rc + p]), cs, sn);
+                y[(rc + p) * Ko + j] = make_float2(fmaf(ar, cs, -(ai * sn)), fmaf(ar, sn, ai * cs));
+            }
         }
     }
 }

@@ -62,10 +62,10 @@ __global__ __launch_bounds__(256) void k_prepare(const float2* __restrict__ S, i
 // h_r = taps[r] e^{2 pi i r sl}: per (row, child) L complex taps and one phasor per output, so no rotated copy of the
 // window is made. A thread forms a run of consecutive taps of one row (one tap where the block has a thread per tap):
 // the run's first ramp value seeded in float64, then stepped by a float32 recurrence; the output phasor in float64.
-// grid (ceil(Ko / JB), ceil(P /
-// PB), Np), block 256; where the block has fewer outputs than threads, SO = 256 / (PB JB) consecutive threads share
-// an output, each summing every SO-th tap, added across them in a fixed order. Dynamic LDS: the window of the
-// block's rows [PB][W] (float2, W = D (JB - 1) + L, loaded once for every child) and the complex taps [PB][L].
+// grid (ceil(Ko / JB), ceil(P / PB), Np), block 256; where the block has fewer outputs than threads, SO = 256 / (PB
+// JB) consecutive threads share an output, each summing every SO-th tap, added across them in a fixed order. Dynamic
+// LDS: the window of the block's rows [PB][W] (float2, W = D (JB - 1) + L, loaded once for every child) and the
+// complex taps [PB][L].
 __global__ __launch_bounds__(256) void k_rot_fir_k(const float2* __restrict__ x, int P, int K, const float* __restrict__ c0,
                                                     const float* __restrict__ sl, int Cn, const float* __restrict__ taps, int L,
                                                     int pl, int D, int Ko, double kc, int JB, int PB, float2* __restrict__ y)

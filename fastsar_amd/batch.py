@@ -6,8 +6,9 @@
     img = b.images[i].to_host()
 
 The batch captures every former's launches (window and NaN scan, the levels in groups, the final stage) into one
-hipGraph, spread round-robin over forked streams so that small images run beside each other; replays launch the
-whole batch at once. No reference counterpart: the reference forms its patches one former call at a time
+hipGraph, spread over forked streams so that small images run beside each other (by default balanced: the images,
+timed one call each, each given in turn, largest first, to the stream with the least time so far); replays launch
+the whole batch at once. No reference counterpart: the reference forms its patches one former call at a time
 (fastsar/patches.py:790-870), which is what `run_loop` does here for comparison.
 """
 import ctypes
@@ -27,7 +28,8 @@ def _bind():
         vp, ci = ctypes.c_void_p, ctypes.c_int
         L.fsh_ffbp_enqueue.argtypes = [vp, vp, vp, vp]
         L.fsh_ffbp_bad_pulse.argtypes = [vp, _hip.P_i32]
-        L.fsh_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ci, ci, ctypes.POINTER(vp)]
+        L.fsh_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ci, ci,
+                                       ctypes.POINTER(ci), ctypes.POINTER(vp)]
         L.fsh_batch_run.argtypes = [vp, _hip.P_f32, _hip.P_f32, _hip.P_i32]
         L.fsh_batch_destroy.argtypes = [vp]
         for n in ('fsh_ffbp_enqueue', 'fsh_ffbp_bad_pulse', 'fsh_batch_create', 'fsh_batch_run', 'fsh_batch_destroy'):
@@ -82,12 +84,28 @@ def autotune(ant, fmin, df, K, nx, ny, spx, spy, e1=(1.0, 0.0, 0.0), e2=(0.0, 1.
     return f
 
 
+def stream_order(cost, ns):
+    """Images with the given costs over ns streams, largest first, each to the stream with the least cost so far
+    (the lowest-numbered on ties). -> (order, stream): the capture order (order[p] = the image captured p-th) and
+    the stream of each captured position."""
+    n = len(cost)
+    order = sorted(range(n), key=lambda i: (-cost[i], i))
+    load, stream = [0.0] * ns, []
+    for i in order:
+        s = min(range(ns), key=lambda k: (load[k], k))
+        load[s] += cost[i]
+        stream.append(s)
+    return order, stream
+
+
 class Batch:
     """One hipGraph over the formers' images: formers (warm ImageFormers without a range gate), histories (device
     complex64 [P, K] each), images (device [nx, ny] each, allocated when None); streams: forked streams the images
-    are spread over. Replayable: each run() forms every image again from its history as it then is."""
+    are spread over; balance: give the images to the streams by their single-call times (largest first, each to the
+    stream with the least time so far) instead of round-robin in the given order. Replayable: each run() forms every
+    image again from its history as it then is."""
 
-    def __init__(self, formers, histories, images=None, streams=4):
+    def __init__(self, formers, histories, images=None, streams=4, balance=True):
         L = _bind()
         n = len(formers)
         if n < 1 or len(histories) != n:
@@ -109,12 +127,25 @@ class Batch:
                 _hip.output(im, (f._form.nx, f._form.ny))
         self.images = list(images) if images is not None else [_hip.DeviceArray((f._form.nx, f._form.ny), np.complex64)
                                                                for f in formers]
+        # the capture order (image order[p] captured p-th) and each position's stream
+        ns = max(1, min(int(streams), n))
+        self.order, stream = list(range(n)), None
+        if balance and n > ns:
+            cost = []                      # each image's launches timed alone (enqueued: a bad history raises nothing)
+            for f, S, im in zip(self.formers, self.histories, self.images):
+                _hip.check(L.fsh_synchronize())
+                t0 = time.perf_counter()
+                _hip.check(L.fsh_ffbp_enqueue(f._form.h, S.ptr, im.ptr, None), 'ffbp_enqueue')
+                _hip.check(L.fsh_synchronize())
+                cost.append(time.perf_counter() - t0)
+            self.order, stream = stream_order(cost, ns)
         vp = ctypes.c_void_p
-        hs = (vp * n)(*[f._form.h for f in formers])
-        ss = (vp * n)(*[S.ptr for S in histories])
-        ims = (vp * n)(*[im.ptr for im in self.images])
+        hs = (vp * n)(*[self.formers[i]._form.h for i in self.order])
+        ss = (vp * n)(*[self.histories[i].ptr for i in self.order])
+        ims = (vp * n)(*[self.images[i].ptr for i in self.order])
         h = vp()
-        _hip.check(L.fsh_batch_create(hs, ss, ims, n, int(streams), ctypes.byref(h)), 'batch_create')
+        so = None if stream is None else (ctypes.c_int * n)(*stream)
+        _hip.check(L.fsh_batch_create(hs, ss, ims, n, int(streams), so, ctypes.byref(h)), 'batch_create')
         self._h = h.value
         self.n = n
 
@@ -126,7 +157,10 @@ class Batch:
         done = (ctypes.c_float * self.n)()
         bad = (ctypes.c_int * self.n)()
         _hip.check(L.fsh_batch_run(ctypes.c_void_p(self._h), ctypes.byref(ms), done, bad), 'batch_run')
-        return dict(ms=ms.value, done_ms=np.array(done[:], np.float64), bad=list(bad[:]))
+        d, b = np.empty(self.n), [0] * self.n
+        for p, i in enumerate(self.order):                         # back to the caller's order
+            d[i], b[i] = done[p], bad[p]
+        return dict(ms=ms.value, done_ms=d, bad=b)
 
     def run_loop(self):
         """The same images formed by one former call each, in order (the reference's way) -> dict(ms, each_ms)."""

@@ -1,7 +1,7 @@
 // Many images in one launch: the launches of n formers (each with its own device history, image and buffers),
-// captured into one hipGraph. The images are spread round-robin over forked streams, so that small images, which
-// fill only part of the chip, run beside each other; a one-thread kernel stamps the device wall clock at the start
-// and after each image, which gives each image's completion time within the batch.
+// captured into one hipGraph. The images are spread over forked streams (round-robin, or as the caller assigns them),
+// so that small images, which fill only part of the chip, run beside each other; a one-thread kernel stamps the
+// device wall clock at the start and after each image, which gives each image's completion time within the batch.
 #include "fsh_common.h"
 #include "fsh_kernels.h"
 #include "../../include/fsh.h"
@@ -52,13 +52,21 @@ using namespace fsh;
 
 extern "C" {
 
-int fsh_batch_create(void* const* formers, const void* const* S, void* const* images, int n, int nstreams, void** batch)
+int fsh_batch_create(void* const* formers, const void* const* S, void* const* images, int n, int nstreams,
+                     const int* stream_of, void** batch)
 {
     if (!formers || !S || !images || !batch || n < 1 || nstreams < 1 || nstreams > 16) {
         set_error("batch_create: needs n >= 1 formers with histories and images, and 1 to 16 streams");
         return E_INVALID;
     }
     *batch = nullptr;
+    const int ns = std::min(nstreams, n);
+    if (stream_of)
+        for (int i = 0; i < n; ++i)
+            if (stream_of[i] < 0 || stream_of[i] >= ns) {
+                set_error("batch_create: image " + std::to_string(i) + "'s stream is not in [0, " + std::to_string(ns) + ")");
+                return E_INVALID;
+            }
     std::set<void*> seen;
     for (int i = 0; i < n; ++i) {
         if (!formers[i] || !S[i] || !images[i]) { set_error("batch_create: null former, history or image"); return E_INVALID; }
@@ -73,7 +81,6 @@ int fsh_batch_create(void* const* formers, const void* const* S, void* const* im
     auto fail = [&](int st) { delete b; return st; };
     int st;
     if ((st = dev_alloc((void**)&b->ts, (size_t)(n + 1) * sizeof(unsigned long long)))) return fail(st);
-    const int ns = std::min(nstreams, n);
     if (hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking) != hipSuccess) { set_error("batch: stream"); return fail(E_RUNTIME); }
     b->side.resize(ns, nullptr);
     for (auto& s : b->side)
@@ -89,7 +96,7 @@ int fsh_batch_create(void* const* formers, const void* const* S, void* const* im
         FSH_TRY(hipEventRecord(b->events[ns], b->origin));
         for (auto s : b->side) FSH_TRY(hipStreamWaitEvent(s, b->events[ns], 0));
         for (int i = 0; i < n; ++i) {
-            hipStream_t s = b->side[i % ns];
+            hipStream_t s = b->side[stream_of ? stream_of[i] : i % ns];
             int r = fsh_ffbp_enqueue(formers[i], S[i], images[i], s);
             if (r) return r;
             hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, b->ts, i + 1);

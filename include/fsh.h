@@ -220,9 +220,11 @@ int fsh_ffbp_destroy(void* handle);
  * at a time per former handle until the stream has run it. */
 int fsh_ffbp_enqueue(void* handle, const void* S, void* image, void* stream);
 int fsh_ffbp_bad_pulse(void* handle, int* bad_pulse);
-/* n formers, each with its device history and image, captured into one hipGraph over nstreams forked streams
- * (image i on stream i % nstreams), with a device wall-clock stamp at the start and after each image. */
-int fsh_batch_create(void* const* formers, const void* const* S, void* const* images, int n, int nstreams, void** batch);
+/* n formers, each with its device history and image, captured in order into one hipGraph over min(nstreams, n)
+ * forked streams (image i on stream stream_of[i], or i % nstreams where stream_of is NULL), with a device
+ * wall-clock stamp at the start and after each image. */
+int fsh_batch_create(void* const* formers, const void* const* S, void* const* images, int n, int nstreams,
+                     const int* stream_of, void** batch);
 /* one launch of the graph, waited for: ms the whole batch (host clock), done_ms [n] (or NULL) each image's completion
  * after the batch's start by the device wall clock; bad [n] (or NULL) each image's first non-finite pulse or -1 */
 int fsh_batch_run(void* batch, float* ms, float* done_ms, int* bad);

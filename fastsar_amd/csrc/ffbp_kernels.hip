@@ -1179,7 +1179,9 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
         return E_INVALID;
     }
     if (NW <= 16 && !(force && strcmp(force, "lds") == 0)) {
-        const int IC = std::min(Po, 128);
+        int IC = 128;                                                    // output pulses per segment
+        if (const char* e = getenv("FSH_FIRP_IC")) IC = std::max(1, atoi(e));          // sweeps
+        IC = std::min(Po, IC);
         dim3 grid((unsigned)(((long)B * Ko + 255) / 256), (Po + IC - 1) / IC);
         const bool batch = force && strcmp(force, "stream4") == 0;
         const float* rc0 = rot ? rot->c0 : nullptr;

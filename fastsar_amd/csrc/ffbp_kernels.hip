@@ -60,8 +60,9 @@ __global__ __launch_bounds__(256) void k_prepare(const float2* __restrict__ S, i
 // The rotation folded into the taps: with k_j = D j - pl,
 //   y_j = sum_r taps[r] x[k_j + r] e^{2 pi i (c0 + (k_j + r - kc) sl)} = e^{2 pi i (c0 + (k_j - kc) sl)} sum_r h_r x[k_j + r],
 // h_r = taps[r] e^{2 pi i r sl}: per (row, child) L complex taps and one phasor per output, so no rotated copy of the
-// window is made. A thread forms a run of consecutive taps of one row (one tap where the block has a thread per tap):
-// the run's first ramp value seeded in float64, then stepped by a float32 recurrence; the output phasor in float64.
+// window is made. For long filters a thread forms a run of consecutive taps of one row, the run's first ramp value
+// seeded in float64, then stepped by a float32 recurrence (each tap seeded in float64 otherwise); the output phasor
+// in float64.
 // grid (ceil(Ko / JB), ceil(P / PB), Np), block 256; where the block has fewer outputs than threads, SO = 256 / (PB
 // JB) consecutive threads share an output, each summing every SO-th tap, added across them in a fixed order. Dynamic
 // LDS: the window of the block's rows [PB][W] (float2, W = D (JB - 1) + L, loaded once for every child) and the
@@ -85,10 +86,11 @@ __global__ __launch_bounds__(256) void k_rot_fir_k(const float2* __restrict__ x,
     for (int c = 0; c < Cn; ++c) {
         const size_t rc = ((size_t)n * Cn + c) * P;
         __syncthreads();                                   // the window is in; the previous child's taps are read
-        // runs of RL consecutive taps of one row a thread (RL = 1 while every tap has a thread of its own): the
-        // run's first ramp value and its step seeded in float64, the rest by a float32 recurrence
+        // runs of RL consecutive taps of one row a thread where a row's taps outnumber the threads three to one
+        // (else one tap an item, each seeded in float64): the run's first ramp value and its step seeded in float64,
+        // the rest by a float32 recurrence
         int RL = 1;
-        while (PB * ((L + RL - 1) / RL) > 256) ++RL;
+        while (PB * L > 3 * 256 && PB * ((L + RL - 1) / RL) > 256) ++RL;     // runs where taps outnumber threads 3:1
         const int NR = (L + RL - 1) / RL;
         for (int e = tid; e < PB * NR; e += 256) {
             const int rr = e / NR, r0 = (e - rr * NR) * RL, r1 = min(L, r0 + RL), p = min(p0 + rr, P - 1);
@@ -112,7 +114,13 @@ __global__ __launch_bounds__(256) void k_rot_fir_k(const float2* __restrict__ x,
             const float2* xs = win + (size_t)rr * W + D * jj;
             const float2* hr = h + (size_t)rr * L;
             float ar = 0.f, ai = 0.f;
-            if (ok) {
+            if (ok && SO == 1) {                                  // unit stride (the common case), kept apart
+                for (int r = 0; r < L; ++r) {
+                    const float2 v = xs[r], t = hr[r];
+                    ar = fmaf(t.x, v.x, fmaf(-t.y, v.y, ar));
+                    ai = fmaf(t.x, v.y, fmaf(t.y, v.x, ai));
+                }
+            } else if (ok) {
                 for (int r = so; r < L; r += SO) {
                     const float2 v = xs[r], t = hr[r];
                     ar = fmaf(t.x, v.x, fmaf(-t.y, v.y, ar));

@@ -870,7 +870,8 @@ __global__ __launch_bounds__(256) void k_final_tiles(const float2* __restrict__ 
 // geometric: A_i = d exp(-2 pi i ux f_q dlx_i) = a_{i0} r^(i - i0) with r = exp(-2 pi i ux spx f_q), and likewise
 // B_j. Per p the block's first entry and the ratio are seeded in float64 at q = 0 and advanced over q by their
 // own ratios (one complex multiply each); per q the block's A and B rows take BI - 1 multiplies each, then BI^2
-// complex multiply-adds. Each pixel sums its (p, q) terms in order. W: aperture weights present.
+// complex multiply-adds. Each pixel sums its (p, q) terms in order; a lane reads its tile's samples one ahead of
+// their use. W: aperture weights present.
 template <int T, bool W>
 __global__ __launch_bounds__(256) void k_final_tiles_reg(const float2* __restrict__ data, int B, int Pf, int Qf,
                                                           const double* __restrict__ ux, const double* __restrict__ uy,
@@ -920,9 +921,12 @@ __global__ __launch_bounds__(256) void k_final_tiles_reg(const float2* __restric
             }
         }
         const float2* dq = dp + (size_t)p * Qf;
+        float2 dn = load_ahead(dq);                    // the next sample, read one ahead
         for (int q = 0; q < Qf; ++q) {
             float2 av[BI], bv[BI];
-            av[0] = cmul(dq[q], ba);
+            const float2 dv = dn;
+            dn = load_ahead(dq + min(q + 1, Qf - 1));
+            av[0] = cmul(dv, ba);
             bv[0] = bb;
             #pragma unroll
             for (int k = 1; k < BI; ++k) {

@@ -4,6 +4,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 namespace fsh {
 
@@ -59,6 +60,16 @@ __device__ __forceinline__ void wave_lds_sync()
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
     return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
+
+// a read of 4 or 8 bytes made as a relaxed agent-scope atomic: the compiler keeps it where it is issued (a plain read
+// it may sink to its first use), so a read issued ahead of the work before its use stays ahead of it
+template <typename V>
+__device__ __forceinline__ V load_ahead(const V* p)
+{
+    static_assert(sizeof(V) == 4 || sizeof(V) == 8, "load_ahead reads 4 or 8 bytes");
+    typedef typename std::conditional<sizeof(V) == 8, unsigned long long, unsigned int>::type U;
+    return __builtin_bit_cast(V, __hip_atomic_load(reinterpret_cast<const U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
 }  // namespace fsh

@@ -89,14 +89,13 @@ __global__ __launch_bounds__(256) void k_level_fused(const float2* __restrict__ 
     // behind the chunk before), the rest when the chunk is staged
     constexpr int XPF = 4;
     float2 xpf[XPF];
+    // (every thread makes the same XPF reads, past the chunk clamped to its last sample, so that the number of
+    // reads in flight does not depend on the path and a wait for an earlier read leaves these in flight)
     auto prefetch = [&](int p0) {
         const float2* src = xn + (size_t)p0 * K;
         const int cnt = min(R, P - p0) * K;
         #pragma unroll
-        for (int k = 0; k < XPF; ++k) {
-            const int i = tid + 256 * k;
-            xpf[k] = i < cnt ? src[i] : make_float2(0.f, 0.f);
-        }
+        for (int k = 0; k < XPF; ++k) xpf[k] = load_ahead(src + min(tid + 256 * k, cnt - 1));
     };
     prefetch(0);
     for (int p0 = 0; p0 < P; p0 += R) {
@@ -118,7 +117,16 @@ __global__ __launch_bounds__(256) void k_level_fused(const float2* __restrict__ 
             }
         }
         __syncthreads();
-        if (p0 + R < P) prefetch(p0 + R);
+        // this thread's range-filter row phase terms, read before the next chunk's prefetch so that waiting for
+        // them leaves the prefetch in flight (both are ordered reads)
+        float ra0, rb0;
+        {
+            const int c = min(tid / R, cn - 1), r = min(tid - (tid / R) * R, rn - 1);   // clamped: every thread reads
+            const size_t ci = ((size_t)n * C + cbeg + c) * P + p0 + r;
+            ra0 = load_ahead(c0 + ci);
+            rb0 = load_ahead(sl + ci);
+        }
+        prefetch(p0 + R < P ? p0 + R : p0);             // the last chunk reads its own rows again, unused
 
         // range filter: thread (c, r)
         if (tid < cn * R) {
@@ -130,8 +138,7 @@ __global__ __launch_bounds__(256) void k_level_fused(const float2* __restrict__ 
                     #pragma unroll
                     for (int s = 0; s < NWK; ++s) hk[ph][s] = ph < Dk ? hk_t[ph * NWK + s] : 0.f;
                 }
-                const size_t ci = ((size_t)n * C + cbeg + c) * P + p0 + r;
-                const double a = (double)c0[ci], b = (double)sl[ci];
+                const double a = (double)ra0, b = (double)rb0;
                 float er, ei, wr = 0.f, wi = 0.f;
                 cis_cyc(b, er, ei);
                 const float2* xr = xs + r * KS;

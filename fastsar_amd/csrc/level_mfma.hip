@@ -376,8 +376,10 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
             // m - NWP + 1 + s of the pulse filter takes tap Dp (NWP - 1 - s) + ph of this row
             const float2 e0 = rl[4 * (wv + kt * NWV) + grp];
             float er = e0.x, eis = t ? e0.y : -e0.y;
+            const int jt0 = 16 * (jr * JT + wv + kt * NWV);       // the tile's first output (uniform)
             #pragma unroll
             for (int i = 0; i < 4; ++i) {
+                if (jt0 + i >= Ko) break;                        // rows i.. of every lane group lie past the last output
                 const float mine = acc[i];
                 const float other = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, mine), 0xB1, 0xF, 0xF, false));
                 const float v = fmaf(mine, er, other * eis);

@@ -448,6 +448,19 @@ __global__ __launch_bounds__(256) void k_fir_p(const float2* __restrict__ y, int
     }
 }
 
+// an output of the pulse filter as stored: the complex value, or (split, where the next level is the one-pass
+// matrix-core level) its bfloat16 x 3 operand words (hi, lo) in the same 8 bytes, split as that level would
+__device__ __forceinline__ void store_out(float2* z, float2 v, int split)
+{
+    if (split) {
+        unsigned int hi, lo;
+        split_pair(v.x, v.y, hi, lo);
+        *reinterpret_cast<uint2*>(z) = make_uint2(hi, lo);
+    } else {
+        *z = v;
+    }
+}
+
 // Streaming form: one thread per column (b, j) and run of IC outputs; the input rows are read once, in order, and
 // each is added into the NW = ceil(L / D) outputs whose taps cover it, held as a sliding window of accumulators. Row
 // p = D m - pl + s (period m, phase s < D) feeds output i with tap D (m - i) + s; at the end of period m output
@@ -459,7 +472,7 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
                                                        const float* __restrict__ taps, int L, int pl, int D, int Po, int IC,
                                                        float2* __restrict__ z, const float* __restrict__ hs,
                                                        const float* __restrict__ rc0, const float* __restrict__ rsl, int rD,
-                                                       int rpl, double rkc, int Ys)
+                                                       int rpl, double rkc, int Ys, int split)
 {
     typedef _Float16 h2 __attribute__((ext_vector_type(2)));
     const float hinv = hs ? hs[1] : 1.f;             // float16 input: its scale undone by hs[1]
@@ -559,7 +572,7 @@ __global__ __launch_bounds__(256) void k_fir_p_stream(const float2* __restrict__
             }
         }
         const int done = m - NW + 1;
-        if (done >= i0 && done < i1) zb[(size_t)done * Ko] = acc[0];
+        if (done >= i0 && done < i1) store_out(zb + (size_t)done * Ko, acc[0], split);
         #pragma unroll
         for (int s = 0; s < NW - 1; ++s) acc[s] = acc[s + 1];
         acc[NW - 1] = make_float2(0.f, 0.f);
@@ -575,7 +588,7 @@ __global__ __launch_bounds__(256) void k_fir_p_stream2(const float2* __restrict_
                                                         const float* __restrict__ taps, int L, int pl, int D, int Po, int IC,
                                                         float2* __restrict__ z, const float* __restrict__ hs,
                                                         const float* __restrict__ rc0, const float* __restrict__ rsl, int rD,
-                                                        int rpl, double rkc, int Ys)
+                                                        int rpl, double rkc, int Ys, int split)
 {
     typedef _Float16 h2 __attribute__((ext_vector_type(2)));
     constexpr int RB = 4;
@@ -652,8 +665,8 @@ __global__ __launch_bounds__(256) void k_fir_p_stream2(const float2* __restrict_
         }
         const int done = m - NW + 1;
         if (done >= i0 && done < i1) {
-            if (one) za[(size_t)done * Ko] = acca[0];
-            if (two) zb2[(size_t)done * Ko] = accb[0];
+            if (one) store_out(za + (size_t)done * Ko, acca[0], split);
+            if (two) store_out(zb2 + (size_t)done * Ko, accb[0], split);
         }
         #pragma unroll
         for (int s = 0; s < NW - 1; ++s) { acca[s] = acca[s + 1]; accb[s] = accb[s + 1]; }
@@ -1173,7 +1186,7 @@ bool fir_p_streams(int L, int D)
 }
 
 int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float2* z, hipStream_t s,
-                 const float* hs, const DeferredRot* rot, int ys)
+                 const float* hs, const DeferredRot* rot, int ys, bool split)
 {
     const int Ys = ys > 0 ? ys : Ko;
     if (B < 1 || P < 1 || Ko < 1 || L < 1 || D < 1 || Po < 1 || pl < 0 || Ys < Ko) {
@@ -1186,10 +1199,12 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
         set_error("fir_p: a float16 input is read by the streaming kernel only (at most 16 taps per phase)");
         return E_INVALID;
     }
-    if ((rot || Ys != Ko) && !fir_p_streams(L, D)) {
-        set_error("fir_p: a deferred range rotation or a padded row stride is taken by the streaming kernel only");
+    if ((rot || Ys != Ko || split) && !fir_p_streams(L, D)) {
+        set_error("fir_p: a deferred range rotation, a padded row stride or split output words are taken by the streaming "
+                  "kernel only");
         return E_INVALID;
     }
+    const int sp = split ? 1 : 0;
     if (NW <= 16 && !(force && strcmp(force, "lds") == 0)) {
         int IC = 128;                                                    // output pulses per segment
         if (const char* e = getenv("FSH_FIRP_IC")) IC = std::max(1, atoi(e));          // sweeps
@@ -1210,7 +1225,7 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
             const int H = (Ys + 1) / 2;
             dim3 grid2((unsigned)(((long)B * H + 255) / 256), (Po + IC - 1) / IC);
 #define FSH_FP2(NN) hipLaunchKernelGGL((k_fir_p_stream2<NN>), grid2, dim3(256), (size_t)4 * D * NW, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs, \
-                                       rc0, rsl, rD, rpl, rkc, Ys)
+                                       rc0, rsl, rD, rpl, rkc, Ys, sp)
             switch (NW) {
                 case 1: FSH_FP2(1); break;   case 2: FSH_FP2(2); break;   case 3: FSH_FP2(3); break;   case 4: FSH_FP2(4); break;
                 case 5: FSH_FP2(5); break;   case 6: FSH_FP2(6); break;   case 7: FSH_FP2(7); break;   case 8: FSH_FP2(8); break;
@@ -1225,13 +1240,13 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
 #define FSH_FP(NN)                                                                                                       \
     do {                                                                                                                 \
         if (batch) hipLaunchKernelGGL((k_fir_p_stream<NN, true, 1>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, \
-                                      hs, rc0, rsl, rD, rpl, rkc, Ys);                                                   \
+                                      hs, rc0, rsl, rD, rpl, rkc, Ys, sp);                                               \
         else if (rows == 8) hipLaunchKernelGGL((k_fir_p_stream<NN, false, 8>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, \
-                                               Po, IC, z, hs, rc0, rsl, rD, rpl, rkc, Ys);                               \
+                                               Po, IC, z, hs, rc0, rsl, rD, rpl, rkc, Ys, sp);                           \
         else if (rows == 4) hipLaunchKernelGGL((k_fir_p_stream<NN, false, 4>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, \
-                                               Po, IC, z, hs, rc0, rsl, rD, rpl, rkc, Ys);                               \
+                                               Po, IC, z, hs, rc0, rsl, rD, rpl, rkc, Ys, sp);                           \
         else hipLaunchKernelGGL((k_fir_p_stream<NN, false, 1>), grid, dim3(256), lds, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs, \
-                                rc0, rsl, rD, rpl, rkc, Ys);                                                             \
+                                rc0, rsl, rD, rpl, rkc, Ys, sp);                                                         \
     } while (0)
         switch (NW) {
             case 1: FSH_FP(1); break;

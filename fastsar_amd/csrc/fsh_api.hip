@@ -312,6 +312,9 @@ struct Former {
         int curbuf = -1;
         long long parents = 1;
         long long first_parent = 0;    // index of the group's first parent at the current level, plan order
+        // the current level's input holds bfloat16 x 3 words (the pulse filter before a one-pass matrix-core level
+        // stores them, so that level stages its rows without splitting them)
+        bool xsplit = false;
         for (size_t l = 0; l < lv.size(); ++l) {
             const Level& L = lv[l];
             const fsh_level_desc& h = L.h;
@@ -338,8 +341,9 @@ struct Former {
             if (L.mfma) {
                 mark_begin();
                 if ((st = launch_level_mfma(cur, (int)parents, h.P, h.K, c0, sl, ch, L.taps_k, h.Lk, h.Dk, h.plk, h.Ko, L.hp_t,
-                                            h.Lp, h.Dp, h.plp, h.Po, h.kc, buf[yb], stream)))
+                                            h.Lp, h.Dp, h.plp, h.Po, h.kc, buf[yb], stream, xsplit)))
                     return st;
+                xsplit = false;
                 mark_end(ST_LMFMA);
                 cur = buf[yb];
                 curbuf = yb;
@@ -396,9 +400,15 @@ struct Former {
             curbuf = yb;
             if (h.Dp > 1) {
                 mark_begin();
+                bool split = false;
+                if (l + 1 < lv.size() && lv[l + 1].mfma && fir_p_streams(h.Lp, h.Dp)) {
+                    const fsh_level_desc& n = lv[l + 1].h;
+                    split = level_mfma_split_input(n.C, n.Lk, n.Dk, n.Ko, n.Lp, n.Dp, n.Po);
+                }
                 if ((st = launch_fir_p(buf[yb], (int)(parents * ch), h.P, h.Ko, L.taps_p, h.Lp, h.plp, h.Dp, h.Po, buf[1 - yb],
-                                       stream, hs, deferred ? &drot : nullptr, ystr)))
+                                       stream, hs, deferred ? &drot : nullptr, ystr, split)))
                     return st;
+                xsplit = split;
                 mark_end(ST_FIRP);
                 cur = buf[1 - yb];
                 curbuf = 1 - yb;

@@ -62,6 +62,18 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
     return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 
+// the bfloat16 x 3 operand split of two values at once into packed words: hi = (bf16 a, bf16 b) (round to nearest
+// even), lo = the bf16 of the remainders a - hi and b - hi
+__device__ __forceinline__ void split_pair(float a, float b, unsigned int& hi, unsigned int& lo)
+{
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    const bf16x2_t h = {(__bf16)a, (__bf16)b};
+    hi = __builtin_bit_cast(unsigned int, h);
+    const float ha = __builtin_bit_cast(float, hi << 16), hb = __builtin_bit_cast(float, hi & 0xffff0000u);
+    const bf16x2_t l = {(__bf16)(a - ha), (__bf16)(b - hb)};
+    lo = __builtin_bit_cast(unsigned int, l);
+}
+
 // a read of 4 or 8 bytes made as a relaxed agent-scope atomic: the compiler keeps it where it is issued (a plain read
 // it may sink to its first use), so a read issued ahead of the work before its use stays ahead of it
 template <typename V>

@@ -37,9 +37,10 @@ struct DeferredRot {
     double kc;
 };
 // hs: the input is float16 pairs, times hs[1] when read; rot: each input sample times rho first; ys: rows of y ys >= Ko
-// samples apart (0: Ko) (all three: the streaming kernel only)
+// samples apart (0: Ko); split: each output stored as its bfloat16 x 3 words (hi, lo) for the one-pass matrix-core
+// level that reads it (all four: the streaming kernel only)
 int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L, int pl, int D, int Po, float2* z,
-                 hipStream_t s, const float* hs = nullptr, const DeferredRot* rot = nullptr, int ys = 0);
+                 hipStream_t s, const float* hs = nullptr, const DeferredRot* rot = nullptr, int ys = 0, bool split = false);
 // whether launch_fir_p takes the streaming kernel for this filter (the one that applies a deferred rho)
 bool fir_p_streams(int L, int D);
 // the rotation and both filters of a level in one pass, for short children (fused_kernels.hip); hk_t and hp_t are the
@@ -55,9 +56,11 @@ int level_mfma_nw(int Lp, int Dp);
 bool level_mfma_shape(int Np, int C, int Lk, int Dk, int Ko, int Lp, int Dp, int Po, int& KS, int& NWp, int& NPH, int& PS,
                       int& JT, int& njr, int& nseg, int& IS, size_t& lds, int& nwp, int& tpw, int& nwv);
 bool level_mfma_preferred(int C, int Lk, int Dk, int Ko, int Lp, int Dp, int Po);
+bool level_mfma_split_input(int C, int Lk, int Dk, int Ko, int Lp, int Dp, int Po);
+// xsplit: x holds each sample as its bfloat16 x 3 words (hi, lo) (launch_fir_p with split), staged without a split
 int launch_level_mfma(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int C, const float* hk,
                       int Lk, int Dk, int plk, int Ko, const float* hp_t, int Lp, int Dp, int plp, int Po, double kc,
-                      float2* z, hipStream_t s);
+                      float2* z, hipStream_t s, bool xsplit = false);
 int launch_level_fused(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int C, const float* hk_t,
                        int Lk, int Dk, int plk, int Ko, const float* hp_t, int Lp, int Dp, int plp, int Po, double kc,
                        float2* z, hipStream_t s);

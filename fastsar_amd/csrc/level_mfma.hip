@@ -63,17 +63,6 @@ __device__ __forceinline__ void cis_lin(float a, float b, float m, float& cs, fl
     sn = __builtin_amdgcn_sinf(fr);
 }
 
-// both halves of a bfloat16 pair split: hi = (bf16 a, bf16 b) packed, lo = the bf16 of the remainders
-__device__ __forceinline__ void split2(float a, float b, unsigned int& hi, unsigned int& lo)
-{
-    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
-    const bf16x2_t h = {(__bf16)a, (__bf16)b};
-    hi = __builtin_bit_cast(unsigned int, h);
-    const float ha = __builtin_bit_cast(float, hi << 16), hb = __builtin_bit_cast(float, hi & 0xffff0000u);
-    const bf16x2_t l = {(__bf16)(a - ha), (__bf16)(b - hb)};
-    lo = __builtin_bit_cast(unsigned int, l);
-}
-
 constexpr int WAVES_MULTI = 8;             // waves of a block whose waves take several tiles
 constexpr int TAB_MAX = 128;               // taps: Lk <= Dk NWp <= 4 ceil(Dk / 4) NWp = 8 (2 KS) <= 128
 
@@ -84,7 +73,9 @@ __host__ __device__ constexpr int ef_items(int tpw) { return tpw == 1 ? 3 : 2; }
 
 // Four waves a SIMD (at most 128 registers) where the pulse filter's accumulators leave room: with a barrier per row,
 // the three waves a SIMD of the unconstrained build left the 12288 x 8192 image's level 2 at 131 ms against 123 ms.
-template <int KS, int NWP, int TPW>
+// XS: x holds each sample as its bfloat16 x 3 words (hi, lo), written so by the pulse filter before this level, and
+// its rows are staged without a split (one-tile-a-wave blocks only)
+template <int KS, int NWP, int TPW, bool XS = false>
 __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1 && NWP <= 10 ? 4 : 1) void k_level_mfma(
     const float2* __restrict__ x, int P, int K, const float* __restrict__ c0, const float* __restrict__ sl, int C,
     const float* __restrict__ hk, int Lk, int Dk, int plk, int Ko, int NWp, int NPH, int PS, int JT, int njr,
@@ -237,7 +228,12 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
         for (int k = 0; k < MAXI; ++k) {
             if (it_lds[k] >= 0) {
                 unsigned int hi, lo;
-                split2(xv[k].x, xv[k].y, hi, lo);
+                if constexpr (XS) {                             // x already holds the words (hi, lo)
+                    hi = __builtin_bit_cast(unsigned int, xv[k].x);
+                    lo = __builtin_bit_cast(unsigned int, xv[k].y);
+                } else {
+                    split_pair(xv[k].x, xv[k].y, hi, lo);
+                }
                 XH[it_lds[k]] = hi;
                 XL[it_lds[k]] = lo;
             }
@@ -254,7 +250,7 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
                 const float h = __int_as_float(e.w);
                 const float gr = h * fmaf(fa.x, fb.x, -(fa.y * fb.y)), gi = h * fmaf(fa.x, fb.y, fa.y * fb.x);
                 unsigned int hi, lo;
-                split2(gr, -gi, hi, lo);                       // column Re y_c: (Re g, -Im g)
+                split_pair(gr, -gi, hi, lo);                       // column Re y_c: (Re g, -Im g)
                 unsigned int* w = reinterpret_cast<unsigned int*>(FH + e.x);
                 w[0] = hi;
                 w[256] = lo;
@@ -486,10 +482,18 @@ bool level_mfma_preferred(int C, int Lk, int Dk, int Ko, int Lp, int Dp, int Po)
     return (force && strcmp(force, "mfma") == 0) || (njr == 1 && tpw == 1) || (rows && atoi(rows) == 1 && tpw == 1);
 }
 
+// whether the level's launch stages split input words (its blocks take one tile a wave)
+bool level_mfma_split_input(int C, int Lk, int Dk, int Ko, int Lp, int Dp, int Po)
+{
+    int KS, NWp, NPH, PS, JT, njr, nseg, IS, nwp, tpw, nwv;
+    size_t lds;
+    return level_mfma_shape(1, C, Lk, Dk, Ko, Lp, Dp, Po, KS, NWp, NPH, PS, JT, njr, nseg, IS, lds, nwp, tpw, nwv) && tpw == 1;
+}
+
 // hp_t: the pulse filter's polyphase table for level_mfma_nw(Lp, Dp) accumulators (launch_polyphase_nw)
 int launch_level_mfma(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int C, const float* hk,
                       int Lk, int Dk, int plk, int Ko, const float* hp_t, int Lp, int Dp, int plp, int Po, double kc,
-                      float2* z, hipStream_t s)
+                      float2* z, hipStream_t s, bool xsplit)
 {
     int KS, NWp, NPH, PS, JT, njr, nseg, IS, nwp, tpw, nwv;
     size_t lds;
@@ -500,13 +504,15 @@ int launch_level_mfma(const float2* x, int Np, int P, int K, const float* c0, co
     }
     const long G = (long)Np * nseg * ((C + CB - 1) / CB) * njr;
     if (G > 2147483647L) { set_error("level_mfma: too many blocks"); return E_INVALID; }
+    if (xsplit && tpw != 1) { set_error("level_mfma: split input words are staged by one-tile-a-wave blocks only"); return E_INVALID; }
     const dim3 grid((unsigned)G), block((unsigned)(64 * nwv));
     const float kcf = (float)kc;
-#define FSH_LM(A, B, T) hipLaunchKernelGGL((k_level_mfma<A, B, T>), grid, block, lds, s, x, P, K, c0, sl, C, hk, Lk, Dk, plk, \
-                                           Ko, NWp, NPH, PS, JT, njr, hp_t, Dp, plp, Lp, Po, IS, nseg, kcf, z)
+#define FSH_LM(A, B, T, X) hipLaunchKernelGGL((k_level_mfma<A, B, T, X>), grid, block, lds, s, x, P, K, c0, sl, C, hk, Lk, Dk, \
+                                              plk, Ko, NWp, NPH, PS, JT, njr, hp_t, Dp, plp, Lp, Po, IS, nseg, kcf, z)
 #define FSH_LM_T(A, B)                                                                                                \
     switch (tpw) {                                                                                                    \
-        case 1: FSH_LM(A, B, 1); break; case 2: FSH_LM(A, B, 2); break; default: FSH_LM(A, B, 3); break;             \
+        case 1: if (xsplit) FSH_LM(A, B, 1, true); else FSH_LM(A, B, 1, false); break;                               \
+        case 2: FSH_LM(A, B, 2, false); break; default: FSH_LM(A, B, 3, false); break;                               \
     }
 #define FSH_LM_P(A)                                                                                                   \
     switch (nwp) {                                                                                                    \

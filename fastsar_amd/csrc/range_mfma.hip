@@ -36,18 +36,6 @@ typedef unsigned int uint4_t __attribute__((ext_vector_type(4)));
 
 constexpr int STG = 16 * 66;               // per-wave transpose buffer (floats): 16 children x 32 outputs x (Re, Im)
 
-// split of two values at once into packed words: hi = (bf16 a, bf16 b), lo = the bf16 of the remainders a - hi,
-// b - hi (each rounded to nearest even)
-__device__ __forceinline__ void split_pair(float a, float b, unsigned int& hi, unsigned int& lo)
-{
-    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
-    const bf16x2_t h = {(__bf16)a, (__bf16)b};
-    hi = __builtin_bit_cast(unsigned int, h);
-    const float ha = __builtin_bit_cast(float, hi << 16), hb = __builtin_bit_cast(float, hi & 0xffff0000u);
-    const bf16x2_t l = {(__bf16)(a - ha), (__bf16)(b - hb)};
-    lo = __builtin_bit_cast(unsigned int, l);
-}
-
 // one pair of contraction indices (kappa, kappa + 1) = taps (r0, r0 + D) of phase ph: their values and the word
 // offset ph PS + u of their first sample in the window planes
 struct PairTab {

@@ -1320,7 +1320,16 @@ int launch_final_geom(const double* fu, const double* fr0, const double* cen, in
 
 int final_split(long tiles, int T)
 {
+    if (const char* e = getenv("FSH_FINAL_SPLIT")) {                 // sweeps: a power of two up to 64
+        const int v = atoi(e);
+        if (v >= 1 && v <= 64 && (v & (v - 1)) == 0) return v;
+    }
+    // lanes a pixel block: up to 16 until the image has 65,536 lanes; 4 where it has 2^20 pixel blocks or more (such
+    // images are formed in several groups, each launch about 2^18 blocks: one lane a block fills the device's two
+    // waves a SIMD 1.5 times and its second half runs one wave a SIMD; the 12,288 x 8,192 image 37.5 -> 33.2 ms,
+    // 2 lanes 33.6; taking 16 lanes up to 2^22 lanes instead slowed the 48-image batch 96.3 -> 99.1 ms)
     const long per = (long)(T / std::min(T, 8)) * (T / std::min(T, 8));
+    if (tiles * per >= (1L << 20)) return 4;
     int sp = 1;
     while (sp < 16 && tiles * per * sp < 65536) sp *= 2;
     return sp;

@@ -401,7 +401,8 @@ struct Former {
             if (h.Dp > 1) {
                 mark_begin();
                 bool split = false;
-                if (l + 1 < lv.size() && lv[l + 1].mfma && fir_p_streams(h.Lp, h.Dp)) {
+                const char* she = getenv("FSH_SPLIT_HANDOFF");            // 0: the complex values (A/B and tests)
+                if (l + 1 < lv.size() && lv[l + 1].mfma && fir_p_streams(h.Lp, h.Dp) && !(she && atoi(she) == 0)) {
                     const fsh_level_desc& n = lv[l + 1].h;
                     split = level_mfma_split_input(n.C, n.Lk, n.Dk, n.Ko, n.Lp, n.Dp, n.Po);
                 }

@@ -250,7 +250,7 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
                 const float h = __int_as_float(e.w);
                 const float gr = h * fmaf(fa.x, fb.x, -(fa.y * fb.y)), gi = h * fmaf(fa.x, fb.y, fa.y * fb.x);
                 unsigned int hi, lo;
-                split_pair(gr, -gi, hi, lo);                       // column Re y_c: (Re g, -Im g)
+                split_pair(gr, -gi, hi, lo);                   // column Re y_c: (Re g, -Im g)
                 unsigned int* w = reinterpret_cast<unsigned int*>(FH + e.x);
                 w[0] = hi;
                 w[256] = lo;

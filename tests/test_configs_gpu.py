@@ -3,7 +3,8 @@
 12,288 x 8,192 image from 16,384 pulses x 8,192 samples (6 degree aperture, 2,000 targets over a 6 km x 4 km scene
 plus white clutter 25 dB down; T 8, four levels, 80 dB filters): error against the float64 device checker on seeded
 256^2 tiles, the checker itself against the float64 direct sum over pulses and samples on the CPU at this geometry,
-and the image of groups of 4 first-level children equal bit for bit to that of groups of 8.
+and the image of groups of 4 first-level children equal bit for bit to that of groups of 8 and to that formed with
+complex values, not split operand words, handed from level 1 to level 2.
 
 48 uneven images (sides 256 to 4,096 px, 512 to 8,192 pulses x 1,024 to 4,096 samples, spotlight and 2-degree
 squint, seed 5): each image's final tile chosen by the predicted final-stage error (T 8, else 4, four levels; no
@@ -54,7 +55,9 @@ def full():
 
 
 @pytest.mark.timeout(600)
-def test_full_image_error_and_groups(full):
+def test_full_image_error_and_groups(full, monkeypatch):
+    """also: level 2 stages its rows from the bfloat16 x 3 words the level-1 pulse filter stores for it; with the
+    complex values handed over instead (FSH_SPLIT_HANDOFF=0) the image is the same bit for bit"""
     col, grid, pos, dS = full
     f = ImageFormer(col.ant, col.fmin, col.df, col.K, **grid, T=8, levels=4, atten=80.0)
     img = _hip.DeviceArray((grid['nx'], grid['ny']), np.complex64)
@@ -69,6 +72,9 @@ def test_full_image_error_and_groups(full):
     f._form.set_group(4)
     f(dS, out=img)
     assert f._form.info()['group'] == 4
+    assert np.array_equal(img.to_host(), a)
+    monkeypatch.setenv('FSH_SPLIT_HANDOFF', '0')
+    f(dS, out=img)
     assert np.array_equal(img.to_host(), a)
 
 

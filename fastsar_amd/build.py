@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, 'csrc')
 OUT = os.path.join(HERE, '_lib')
-SOURCES = ['fsh_api.hip', 'host_stage.hip', 'ffbp_kernels.hip', 'range_mfma.hip', 'level_mfma.hip', 'fused_kernels.hip', 'fused_kernels_d2.hip', 'exact_kernels.hip', 'gate_kernels.hip', 'batch.hip', 'learned_kernels.hip', 'sim_kernels.hip']
+SOURCES = ['fsh_api.hip', 'host_stage.hip', 'ffbp_kernels.hip', 'range_mfma.hip', 'level_mfma.hip', 'fused_kernels.hip', 'fused_kernels_k2p2.hip', 'fused_kernels_k2p3.hip', 'fused_kernels_k3p2.hip', 'fused_kernels_k3p3.hip', 'exact_kernels.hip', 'gate_kernels.hip', 'batch.hip', 'learned_kernels.hip', 'sim_kernels.hip']
 ARCH = os.environ.get('FSH_ARCH', 'gfx950')
 
 

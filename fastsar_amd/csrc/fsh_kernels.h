@@ -61,10 +61,17 @@ bool level_mfma_split_input(int C, int Lk, int Dk, int Ko, int Lp, int Dp, int P
 int launch_level_mfma(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int C, const float* hk,
                       int Lk, int Dk, int plk, int Ko, const float* hp_t, int Lp, int Dp, int plp, int Po, double kc,
                       float2* z, hipStream_t s, bool xsplit = false);
-// the same with the range decimation fixed at 2 (fused_kernels_d2.hip), on a shape launch_level_fused has checked
-int launch_level_fused_d2(dim3 grid, size_t lds, hipStream_t s, int NWK, int NWP, const float2* x, int P, int K,
-                          const float* c0, const float* sl, int C, const float* hk_t, int plk, int Ko, const float* hp_t,
-                          int Dp, int plp, int Po, double kc, int Np, int CB, int R, float2* z);
+// the same with the decimation factors fixed at (Dk, Dp) = (2, 2), (2, 3), (3, 2), (3, 3) (fused_kernels_k<Dk>p<Dp>.hip),
+// on a shape launch_level_fused has checked
+#define FSH_FUSED_FIXED_DECL(name)                                                                                    \
+    int name(dim3 grid, size_t lds, hipStream_t s, int NWK, int NWP, const float2* x, int P, int K, const float* c0,    \
+             const float* sl, int C, const float* hk_t, int plk, int Ko, const float* hp_t, int plp, int Po, double kc, \
+             int Np, int CB, int R, float2* z);
+FSH_FUSED_FIXED_DECL(launch_level_fused_k2p2)
+FSH_FUSED_FIXED_DECL(launch_level_fused_k2p3)
+FSH_FUSED_FIXED_DECL(launch_level_fused_k3p2)
+FSH_FUSED_FIXED_DECL(launch_level_fused_k3p3)
+#undef FSH_FUSED_FIXED_DECL
 int launch_level_fused(const float2* x, int Np, int P, int K, const float* c0, const float* sl, int C, const float* hk_t,
                        int Lk, int Dk, int plk, int Ko, const float* hp_t, int Lp, int Dp, int plp, int Po, double kc,
                        float2* z, hipStream_t s);

@@ -116,10 +116,13 @@ int launch_level_fused(const float2* x, int Np, int P, int K, const float* c0, c
     const long G = (long)Np * ((C + CB - 1) / CB);
     if (G > 2147483647L) { set_error("level_fused: too many parents"); return E_INVALID; }
     const dim3 grid((unsigned)G), block(256);
-    const char* dke = getenv("FSH_FUSED_DK");                          // 0: the run-time form also where Dk = 2 (A/B)
-    if (Dk == 2 && !(dke && atoi(dke) == 0))
-        return launch_level_fused_d2(grid, lds, s, NWK, NWP, x, P, K, c0, sl, C, hk_t, plk, Ko, hp_t, Dp, plp, Po, kc, Np, CB,
-                                     R, z);
+    // the forms with both factors fixed where they cover the pair (FSH_FUSED_FIXED=0: the run-time form, for A/B)
+    const char* fxe = getenv("FSH_FUSED_FIXED");
+    if (!(fxe && atoi(fxe) == 0) && (Dk == 2 || Dk == 3) && (Dp == 2 || Dp == 3)) {
+        auto f = Dk == 2 ? (Dp == 2 ? launch_level_fused_k2p2 : launch_level_fused_k2p3)
+                         : (Dp == 2 ? launch_level_fused_k3p2 : launch_level_fused_k3p3);
+        return f(grid, lds, s, NWK, NWP, x, P, K, c0, sl, C, hk_t, plk, Ko, hp_t, plp, Po, kc, Np, CB, R, z);
+    }
 #define FSH_LF(A, B) hipLaunchKernelGGL((k_level_fused<A, B>), grid, block, lds, s, x, P, K, c0, sl, C, hk_t, Dk, plk, Ko, \
                                         hp_t, Dp, plp, Po, kc, Np, CB, R, z)
 #define FSH_LF_P(A)                                                                                                   \

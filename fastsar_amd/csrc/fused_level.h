@@ -1,5 +1,5 @@
 // The one-pass VALU level's kernel template, shared by the translation units that instantiate it
-// (fused_kernels.hip: the decimation factor along range read at run time; fused_kernels_d2.hip: fixed at 2).
+// (fused_kernels.hip: the decimation factors read at run time; fused_kernels_k<Dk>p<Dp>.hip: both fixed).
 #pragma once
 #include "fsh_common.h"
 
@@ -19,9 +19,9 @@ __device__ __forceinline__ void static_for(F&& f)
     }
 }
 
-// DK: the range decimation factor fixed at compile time (its loops over phases exact), or 0: Dk read at run time,
-// the loops running to DMAX with the phases past Dk skipped
-template <int NWK, int NWP, int DK = 0>
+// DK, DP: the range and pulse decimation factors fixed at compile time (the loops over phases exact, half the taps
+// or fewer in scalar registers), or 0: read at run time, the loops running to DMAX with the phases past it skipped
+template <int NWK, int NWP, int DK = 0, int DP = 0>
 __global__ __launch_bounds__(256) void k_level_fused(const float2* __restrict__ x, int P, int K, const float* __restrict__ c0,
                                                       const float* __restrict__ sl, int C, const float* __restrict__ hk_t,
                                                       int Dk, int plk, int Ko, const float* __restrict__ hp_t, int Dp, int plp,
@@ -183,17 +183,18 @@ __global__ __launch_bounds__(256) void k_level_fused(const float2* __restrict__ 
         __syncthreads();
 
         // pulse filter: thread (c, j) over the chunk's rows
-        float hp[DMAX][NWP];
+        constexpr int PL = DP ? DP : DMAX;
+        float hp[PL][NWP];
         #pragma unroll
-        for (int ph = 0; ph < DMAX; ++ph) {
+        for (int ph = 0; ph < PL; ++ph) {
             #pragma unroll
-            for (int s = 0; s < NWP; ++s) hp[ph][s] = ph < Dp ? hp_t[ph * NWP + s] : 0.f;
+            for (int s = 0; s < NWP; ++s) hp[ph][s] = (DP || ph < Dp) ? hp_t[ph * NWP + s] : 0.f;
         }
         const float2* yc = ys + (size_t)pc * R * KoS + pj;
         for (int r = 0; r < rn; ++r) {
             const float2 v = colok ? yc[r * KoS] : make_float2(0.f, 0.f);
             #pragma unroll
-            for (int ph = 0; ph < DMAX; ++ph) {
+            for (int ph = 0; ph < PL; ++ph) {
                 if (ph == pph) {
                     #pragma unroll
                     for (int s = 0; s < NWP; ++s) {
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(256) void k_level_fused(const float2* __restrict__ 
                     }
                 }
             }
-            if (++pph == Dp) { end_period(); pph = 0; }
+            if (++pph == (DP ? DP : Dp)) { end_period(); pph = 0; }
         }
         // the next chunk's x load writes only xs, which no thread reads after the barrier above; its range filter
         // writes ys, which the barrier after that load orders after every thread's pulse filter of this chunk

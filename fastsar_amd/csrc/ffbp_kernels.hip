@@ -593,8 +593,15 @@ __global__ __launch_bounds__(256) void k_fir_p_stream2(const float2* __restrict_
     typedef _Float16 h2 __attribute__((ext_vector_type(2)));
     constexpr int RB = 4;
     const float hinv = hs ? hs[1] : 1.f;
-    extern __shared__ float tpz[];                   // [D * NW], zero beyond L
-    for (int r = threadIdx.x; r < D * NW; r += 256) tpz[r] = r < L ? taps[r] : 0.f;
+    // the taps by phase, [D][NW4] (NW4 = NW rounded up to 4, zero beyond L): a row's NW taps are NW4 / 4 16-byte
+    // reads that every lane makes alike (broadcast), not NW 4-byte ones
+    constexpr int NW4 = (NW + 3) / 4 * 4;
+    extern __shared__ float4 tpz4[];
+    float* tpz = reinterpret_cast<float*>(tpz4);
+    for (int i = threadIdx.x; i < D * NW4; i += 256) {
+        const int ph = i / NW4, s = i - ph * NW4, r = D * (NW - 1 - s) + ph;
+        tpz[i] = (s < NW && r < L) ? taps[r] : 0.f;
+    }
     __syncthreads();
     const int H = (Ys + 1) / 2;
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
@@ -653,9 +660,15 @@ __global__ __launch_bounds__(256) void k_fir_p_stream2(const float2* __restrict_
                     a = rot(a, rb[u], ra[u], rja);
                     c = rot(c, rb[u], ra[u], rjb);
                 }
+                float hv[NW4];
+                #pragma unroll
+                for (int q4 = 0; q4 < NW4 / 4; ++q4) {
+                    const float4 t4 = tpz4[ph * (NW4 / 4) + q4];
+                    hv[4 * q4] = t4.x; hv[4 * q4 + 1] = t4.y; hv[4 * q4 + 2] = t4.z; hv[4 * q4 + 3] = t4.w;
+                }
                 #pragma unroll
                 for (int s = 0; s < NW; ++s) {
-                    const float h = tpz[D * (NW - 1 - s) + ph];
+                    const float h = hv[s];
                     acca[s].x = fmaf(h, a.x, acca[s].x);
                     acca[s].y = fmaf(h, a.y, acca[s].y);
                     accb[s].x = fmaf(h, c.x, accb[s].x);
@@ -1224,7 +1237,7 @@ int launch_fir_p(const float2* y, int B, int P, int Ko, const float* taps, int L
         if (!batch && cols == 2) {
             const int H = (Ys + 1) / 2;
             dim3 grid2((unsigned)(((long)B * H + 255) / 256), (Po + IC - 1) / IC);
-#define FSH_FP2(NN) hipLaunchKernelGGL((k_fir_p_stream2<NN>), grid2, dim3(256), (size_t)4 * D * NW, s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs, \
+#define FSH_FP2(NN) hipLaunchKernelGGL((k_fir_p_stream2<NN>), grid2, dim3(256), (size_t)16 * D * ((NN + 3) / 4), s, y, B, P, Ko, taps, L, pl, D, Po, IC, z, hs, \
                                        rc0, rsl, rD, rpl, rkc, Ys, sp)
             switch (NW) {
                 case 1: FSH_FP2(1); break;   case 2: FSH_FP2(2); break;   case 3: FSH_FP2(3); break;   case 4: FSH_FP2(4); break;

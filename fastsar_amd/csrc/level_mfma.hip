@@ -243,10 +243,7 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
         if (tok) {
             char* FH = reinterpret_cast<char*>(frag + (size_t)b * KS * 2 * 64) + 32 * tc;
             const char* E = reinterpret_cast<const char*>(ef + (b * CB + tc) * NE);
-            #pragma unroll 1
-            for (int i = tt >> 3; i < Lk; i += nthr >> 3) {
-                const int4 e = tapt[i];
-                const float2 fa = *reinterpret_cast<const float2*>(E + e.y), fb = *reinterpret_cast<const float2*>(E + e.z);
+            auto put_tap = [&](const int4 e, const float2 fa, const float2 fb) {
                 const float h = __int_as_float(e.w);
                 const float gr = h * fmaf(fa.x, fb.x, -(fa.y * fb.y)), gi = h * fmaf(fa.x, fb.y, fa.y * fb.x);
                 unsigned int hi, lo;
@@ -257,6 +254,21 @@ __global__ __launch_bounds__(TPW == 1 ? 64 * JT_MAX : 64 * WAVES_MULTI, TPW == 1
                 // column Im y_c: (Im g, Re g), the same halves swapped with the sign of Im restored
                 w[4] = __builtin_amdgcn_alignbit(hi, hi, 16) ^ 0x8000u;
                 w[260] = __builtin_amdgcn_alignbit(lo, lo, 16) ^ 0x8000u;
+            };
+            // two taps a step, their table entries and phasors read together: one LDS round trip a pair, not two
+            const int ts = nthr >> 3;
+            int i = tt >> 3;
+            #pragma unroll 1
+            for (; i + ts < Lk; i += 2 * ts) {
+                const int4 e = tapt[i], e2 = tapt[i + ts];
+                const float2 fa = *reinterpret_cast<const float2*>(E + e.y), fb = *reinterpret_cast<const float2*>(E + e.z);
+                const float2 fa2 = *reinterpret_cast<const float2*>(E + e2.y), fb2 = *reinterpret_cast<const float2*>(E + e2.z);
+                put_tap(e, fa, fb);
+                put_tap(e2, fa2, fb2);
+            }
+            if (i < Lk) {
+                const int4 e = tapt[i];
+                put_tap(e, *reinterpret_cast<const float2*>(E + e.y), *reinterpret_cast<const float2*>(E + e.z));
             }
         }
         #pragma unroll

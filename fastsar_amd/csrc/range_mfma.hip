@@ -78,8 +78,9 @@ __global__ __launch_bounds__(256, 2) void k_rot_fir_k_mf2(const float2* __restri
     const float2* xrow = x + ((size_t)n * P + p) * K;
     // the window: samples ks + i for i < D WM, read in order (coalesced); sample i is word m = i / D of plane
     // ph = i mod D, (m, ph) stepped by 256 samples at a time; XB loads issued before the first is split and stored
+    // (12: a level-1 row's window, about 12 samples a thread, in one round of loads)
     {
-        constexpr int XB = 6;
+        constexpr int XB = 12;
         const int total = D * WM, dm = 256 / D, dph = 256 - dm * D;
         int m = tid / D, ph = tid - m * D;
         for (int i0 = tid; i0 < total; i0 += 256 * XB) {
